@@ -1,0 +1,341 @@
+"""Host-side mirror of the reference's collision / spatial-operator / stepper API.
+
+The reference keeps its grid and A-arrays in the ``commvar`` module and exposes
+Fortran procedures that read them (alexmalekseenko/DGV0D3V_direct_collision):
+
+=====================================  =================================================
+reference                              here (``DGV`` instance method)
+=====================================  =================================================
+SetNodesDGV / SetCellsUGINdsSftArrs    ``DGV(M, s, box)`` (grid + canonical cell)
+ReadAarraysChnksOfstNrecDGV            ``set_aarray`` / ``read_aarray``
+SetA_DGV                               ``generate_aarray``
+WriteAarraysDGV                        ``write_aarray``
+EvalCollisionPeriodicAPlus_DGV(f,fc)   ``EvalCollisionPeriodicAPlus_DGV(f)``
+EvalCollisionPeriodicA_DGV(f,fc)       ``EvalCollisionPeriodicA_DGV(f)``
+SpatialOperSH_DGV(f,fcol,dt)           ``SpatialOperSH_DGV(f, dt)``
+TimeIntegratorORK_SH_DGV(f,dt)         ``TimeIntegratorORK_SH_DGV(f, dt)`` (f in place)
+TimeIntegratorMTS_SH_DGV(f,dt)         ``TimeIntegratorMTS_SH_DGV(f, dt)`` (f in place)
+PrepareMTS_SH_DGV(f,dt)                ``PrepareMTS_SH_DGV(f, dt)``
+MassCheckRec(f,n,u,v,w,T)              ``MassCheckRec(f)``
+=====================================  =================================================
+
+Errors surface as ``DGVError`` where the reference prints and STOPs.  All
+arithmetic runs in libdgv_amd.so on the GPU; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import AgenParams, DGVError, GridDesc, check, dptr, iptr, load
+
+
+def grid_desc(M, s=3, box=(-1.0, 1.0)) -> GridDesc:
+    if np.isscalar(M):
+        M = (M, M, M)
+    if np.isscalar(s):
+        s = (s, s, s)
+    if len(box) == 2:
+        box = (box[0], box[1]) * 3
+    return GridDesc(int(M[0]), int(M[1]), int(M[2]), int(s[0]), int(s[1]), int(s[2]),
+                    *[float(b) for b in box])
+
+
+def agen_pair(M, s, box, i2: int, i3: int, Trad=100.0, min_sens=0.0, nquad=24,
+              moment_mode=0):
+    """Host evaluation of one pair's Atemp_l and A_l (no GPU needed)."""
+    L = load()
+    g = grid_desc(M, s, box)
+    D = g.su * g.sv * g.sw
+    raw = np.zeros(D)
+    A = np.zeros(D)
+    p = AgenParams(Trad, min_sens, nquad, moment_mode)
+    rc = L.dgv_agen_pair(C.byref(g), i2, i3, C.byref(p), dptr(raw), dptr(A))
+    check(rc)
+    return raw, A, rc == 1
+
+
+def grid_geometry(M, s=3, box=(-1.0, 1.0)):
+    """(u, v, w, gwts, ccell) of a grid, computed on the host (no GPU needed)."""
+    L = load()
+    g = grid_desc(M, s, box)
+    n = C.c_int64()
+    check(L.dgv_grid_geometry(C.byref(g), None, None, None, None, None, C.byref(n)))
+    u, v, w, gw = (np.zeros(n.value) for _ in range(4))
+    cc = C.c_int32()
+    check(L.dgv_grid_geometry(C.byref(g), dptr(u), dptr(v), dptr(w), dptr(gw), C.byref(cc),
+                              C.byref(n)))
+    return u, v, w, gw, cc.value
+
+
+def chunk_name(M, s, box, directory="", base="", chunk=0, k_c=2, N_x=3) -> str:
+    """MakeBaseNameAoperChnksDGV file name (readwrite.f90:2224-2281) + "_Aarrs.dat"."""
+    g = grid_desc(M, s, box)
+    buf = C.create_string_buffer(1024)
+    check(load().dgv_aarray_chunk_name(C.byref(g), directory.encode(), base.encode(), chunk,
+                                       k_c, N_x, buf, 1024))
+    return buf.value.decode()
+
+
+class DGV:
+    """One velocity grid, its A-array and the evaluator state on one GPU."""
+
+    def __init__(self, M, s=3, box=(-1.0, 1.0), device: int = 0, mol_diam=1.0, L_inf=1.0,
+                 N_inf=1.0, rkmts: int = 2):
+        self._L = load()
+        self.desc = grid_desc(M, s, box)
+        p = C.c_void_p()
+        rc = self._L.dgv_ctx_create(C.byref(self.desc), device, C.byref(p))
+        check(rc)
+        self._ctx = p
+        n = C.c_int64()
+        check(self._L.dgv_ctx_num_nodes(self._ctx, C.byref(n)), self._ctx)
+        self.nnodes = n.value
+        self.D = self.desc.su * self.desc.sv * self.desc.sw
+        self.M = (self.desc.Mu, self.desc.Mv, self.desc.Mw)
+        cc = C.c_int32()
+        check(self._L.dgv_ctx_canonical_cell(self._ctx, C.byref(cc)), self._ctx)
+        self.ccell = cc.value
+        self.nodes_u = np.zeros(self.nnodes)
+        self.nodes_v = np.zeros(self.nnodes)
+        self.nodes_w = np.zeros(self.nnodes)
+        self.nodes_gwts = np.zeros(self.nnodes)
+        check(self._L.dgv_ctx_get_nodes(self._ctx, dptr(self.nodes_u), dptr(self.nodes_v),
+                                        dptr(self.nodes_w), dptr(self.nodes_gwts)), self._ctx)
+        # (mol_diam/L_inf)**2*N_inf of SpatialOperSH_DGV (spat_oper_mod.f90:65)
+        self.coef = (mol_diam / L_inf) ** 2 * N_inf
+        self.rkmts = rkmts
+        self.frhs1 = np.zeros(self.nnodes)
+        self.hist = np.zeros((9, self.nnodes))
+        self.e_time = 0.0
+
+    # -- lifetime ----------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.dgv_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def _chk(self, rc):
+        check(rc, self._ctx)
+
+    def _f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.shape != (self.nnodes,):
+            raise DGVError(-1, f"array of shape {f.shape}, expected ({self.nnodes},)")
+        return f
+
+    # -- geometry ------------------------------------------------------------------
+    def shifts(self):
+        n = self.nnodes
+        ph, du, dv, dw = (np.zeros(n, np.int32) for _ in range(4))
+        self._chk(self._L.dgv_ctx_get_shifts(self._ctx, iptr(ph), iptr(du), iptr(dv), iptr(dw)))
+        return ph, du, dv, dw
+
+    # -- A-array -------------------------------------------------------------------
+    def set_aarray(self, capphi, A, A_xi, A_xi1, index_base: int = 1):
+        capphi = np.ascontiguousarray(capphi, np.int32)
+        A = np.ascontiguousarray(A, np.float64)
+        xi = np.ascontiguousarray(A_xi, np.int32)
+        xi1 = np.ascontiguousarray(A_xi1, np.int32)
+        if not (len(A) == len(xi) == len(xi1)):
+            raise DGVError(-1, "A, A_xi and A_xi1 must have equal length")
+        self._chk(self._L.dgv_ctx_set_aarray(self._ctx, iptr(capphi), len(capphi), dptr(A),
+                                             iptr(xi), iptr(xi1), len(A), index_base))
+
+    def generate_aarray(self, Trad=100.0, min_sens=0.0, nquad=24, moment_mode=0):
+        p = AgenParams(Trad, min_sens, nquad, moment_mode)
+        self._chk(self._L.dgv_ctx_generate_aarray(self._ctx, C.byref(p)))
+
+    def aarray_size(self):
+        n = C.c_int64()
+        per = np.zeros(self.D, np.int64)
+        self._chk(self._L.dgv_ctx_aarray_size(self._ctx, C.byref(n),
+                                              per.ctypes.data_as(C.POINTER(C.c_int64))))
+        return n.value, per
+
+    def workload(self):
+        """dict(records, pairs, visits, pair_visits) of one Q evaluation."""
+        out = np.zeros(4, np.int64)
+        self._chk(self._L.dgv_ctx_workload(self._ctx, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(records=int(out[0]), pairs=int(out[1]), visits=int(out[2]),
+                    pair_visits=int(out[3]))
+
+    def get_aarray_basis(self, l: int, index_base: int = 1):
+        n = C.c_int64()
+        self._chk(self._L.dgv_ctx_get_aarray_basis(self._ctx, l, None, None, None, 0,
+                                                   C.byref(n), index_base))
+        A = np.zeros(n.value)
+        xi = np.zeros(n.value, np.int32)
+        xi1 = np.zeros(n.value, np.int32)
+        self._chk(self._L.dgv_ctx_get_aarray_basis(self._ctx, l, dptr(A), iptr(xi), iptr(xi1),
+                                                   n.value, C.byref(n), index_base))
+        return A, xi, xi1
+
+    def get_aarray(self, index_base: int = 1):
+        """The whole A-array in the reference layout (capphi, A, A_xi, A_xi1)."""
+        capphi = np.zeros(self.nnodes, np.int32)
+        As, xs, x1s = [], [], []
+        for l in range(self.D):
+            A, xi, xi1 = self.get_aarray_basis(l, index_base)
+            capphi[(self.ccell - 1) * self.D + l] = len(A)
+            As.append(A)
+            xs.append(xi)
+            x1s.append(xi1)
+        return capphi, np.concatenate(As), np.concatenate(xs), np.concatenate(x1s)
+
+    def write_aarray(self, path: str):
+        self._chk(self._L.dgv_aarray_write_file(self._ctx, path.encode()))
+
+    def read_aarray(self, path: str):
+        self._chk(self._L.dgv_ctx_read_aarray_file(self._ctx, path.encode()))
+
+    def set_kernel(self, kernel: int):
+        self._chk(self._L.dgv_ctx_set_kernel(self._ctx, kernel))
+
+    def set_tile(self, pairs_per_item: int):
+        self._chk(self._L.dgv_ctx_set_tile(self._ctx, pairs_per_item))
+
+    def autotune(self):
+        best = C.c_int32()
+        ms = C.c_double()
+        self._chk(self._L.dgv_ctx_autotune(self._ctx, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
+    # -- the reference's procedures ---------------------------------------------
+    def EvalCollisionPeriodicAPlus_DGV(self, f) -> np.ndarray:
+        """collision_mod.f90:143-229"""
+        f = self._f(f)
+        fc = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_eval_q(self._ctx, dptr(f), dptr(fc), self.nnodes))
+        return fc
+
+    def EvalCollisionPeriodicA_DGV(self, f) -> np.ndarray:
+        """collision_mod.f90:29-129 (same quantity as the APlus variant)"""
+        f = self._f(f)
+        fc = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_eval_q_a(self._ctx, dptr(f), dptr(fc), self.nnodes))
+        return fc
+
+    def SpatialOperSH_DGV(self, f, dt: float) -> np.ndarray:
+        """spat_oper_mod.f90:28-68, run_mode 0"""
+        f = self._f(f)
+        fcol = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_spatial_oper(self._ctx, dptr(f), dptr(fcol), self.nnodes, dt,
+                                           self.coef))
+        return fcol
+
+    def TimeIntegratorORK_SH_DGV(self, f: np.ndarray, dt: float) -> np.ndarray:
+        """time_integr_mod.f90:26-152: advances f in place (f must be float64,
+        contiguous), leaves frhs1 = first stage / dt."""
+        if f.dtype != np.float64 or not f.flags.c_contiguous or f.shape != (self.nnodes,):
+            raise DGVError(-1, "f must be a contiguous float64 array of length nnodes")
+        self._chk(self._L.dgv_rk_step(self._ctx, dptr(f), self.nnodes, dt, self.coef,
+                                      self.rkmts, dptr(self.frhs1)))
+        return f
+
+    def TimeIntegratorMTS_SH_DGV(self, f: np.ndarray, dt: float) -> np.ndarray:
+        """time_integr_mod.f90:314-364 (Adams-Bashforth rkmts), advances e_time."""
+        if f.dtype != np.float64 or not f.flags.c_contiguous or f.shape != (self.nnodes,):
+            raise DGVError(-1, "f must be a contiguous float64 array of length nnodes")
+        self.hist[4] = self.frhs1
+        self._chk(self._L.dgv_ab_step(self._ctx, dptr(f), self.nnodes, dt, self.coef, self.rkmts,
+                                      dptr(self.hist)))
+        self.frhs1 = self.hist[4].copy()
+        self.e_time += dt
+        return f
+
+    def PrepareMTS_SH_DGV(self, f: np.ndarray, dt: float) -> np.ndarray:
+        """time_integr_mod.f90:443-569: rkmts-1 Runge-Kutta steps of order rkmts,
+        pushing f and frhs1 down the multistep history before each (lines 536-548)."""
+        k = self.rkmts
+        h = self.hist  # f1..f4 = h[0..3], frhs1..frhs5 = h[4..8]
+        for _ in range(2, k + 1):
+            h[4] = self.frhs1
+            if k == 2:
+                h[0] = f
+            elif k == 3:
+                h[1] = h[0]; h[0] = f; h[5] = h[4]
+            elif k == 4:
+                h[2] = h[1]; h[1] = h[0]; h[0] = f; h[6] = h[5]; h[5] = h[4]
+            elif k == 5:
+                h[3] = h[2]; h[2] = h[1]; h[1] = h[0]; h[0] = f
+                h[7] = h[6]; h[6] = h[5]; h[5] = h[4]
+            self.TimeIntegratorORK_SH_DGV(f, dt)
+            h[4] = self.frhs1
+            self.e_time += dt
+        return f
+
+    def MassCheckRec(self, f):
+        """dgvtools_mod.f90:1437-1458 -> (n, ubar, vbar, wbar, temp)"""
+        f = self._f(f)
+        out = np.zeros(5)
+        self._chk(self._L.dgv_moments(self._ctx, dptr(f), self.nnodes, dptr(out)))
+        return tuple(out)
+
+    # -- additions --------------------------------------------------------------------
+    def collision_moments(self, q):
+        """sum_i w_i psi_k(v_i) q_i for psi = 1, u, v, w, |u|^2"""
+        q = self._f(q)
+        out = np.zeros(5)
+        self._chk(self._L.dgv_collision_moments(self._ctx, dptr(q), self.nnodes, dptr(out)))
+        return out
+
+    def project_invariants(self, q) -> np.ndarray:
+        q = self._f(q).copy()
+        self._chk(self._L.dgv_project_invariants(self._ctx, dptr(q), self.nnodes))
+        return q
+
+    # -- device-resident use (bench, graphs) -----------------------------------------
+    def device_alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._chk(self._L.dgv_device_alloc(self._ctx, nbytes, C.byref(p)))
+        return p.value
+
+    def device_free(self, ptr: int):
+        self._chk(self._L.dgv_device_free(self._ctx, C.c_void_p(ptr)))
+
+    def htod(self, dst: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self._chk(self._L.dgv_memcpy_htod(self._ctx, C.c_void_p(dst), a.ctypes.data_as(C.c_void_p),
+                                          a.nbytes))
+
+    def dtoh(self, a: np.ndarray, src: int):
+        self._chk(self._L.dgv_memcpy_dtoh(self._ctx, a.ctypes.data_as(C.c_void_p), C.c_void_p(src),
+                                          a.nbytes))
+
+    def eval_q_dev(self, d_f: int, d_q: int):
+        self._chk(self._L.dgv_eval_q_dev(self._ctx, C.c_void_p(d_f), C.c_void_p(d_q), self.nnodes))
+
+    def rk_run_dev(self, d_f: int, dt: float, nsteps: int, rkmts: Optional[int] = None,
+                   use_graph: bool = True):
+        self._chk(self._L.dgv_rk_run_dev(self._ctx, C.c_void_p(d_f), self.nnodes, dt, self.coef,
+                                         rkmts or self.rkmts, nsteps, int(use_graph)))
+
+    def synchronize(self):
+        self._chk(self._L.dgv_ctx_synchronize(self._ctx))
+
+    def time_eval_q_dev(self, d_f: int, d_q: int, niter: int):
+        a = C.c_double()
+        b = C.c_double()
+        self._chk(self._L.dgv_time_eval_q_dev(self._ctx, C.c_void_p(d_f), C.c_void_p(d_q), niter,
+                                              C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -1,0 +1,177 @@
+// dgv_internal.h -- internal types of the MI355X direct DGV collision library.
+//
+// Device layout of the A-array ("pair-block" layout)
+// ---------------------------------------------------
+// The reference stores, per canonical basis function l, a list of records
+// (A, xi, xi1) (commvar.f90:157-167) and the evaluator re-reads the whole list
+// once per node (collision_mod.f90:186-227).  Every record of every l belongs to
+// an unordered node pair (xi, xi1); the geometric prefilter that decides which
+// pairs exist (dgvtools_mod.f90:493-530) does not depend on l.  On the device the
+// records are therefore regrouped by pair:
+//
+//   pair   = (q, T[0..D-1])        T[l] = A of basis l (0 when l has no record),
+//                                   q = local index of xi1 in its cell;
+//   block  = all pairs whose xi lies in cell (cc + a) and xi1 in cell (cc + b),
+//            sorted by p = local index of xi; seg[p] = first pair with index >= p;
+//   group  = all blocks sharing the same box of output cells c for which both
+//            c + a and c + b stay on the grid (the evaluator's bounds tests,
+//            collision_mod.f90:196-211, hold for exactly that box).
+//
+// One Q evaluation then computes, for every output cell c and basis l,
+//   Q[c,l] = (2/w) sum_blocks sum_pairs T[l] * f[c+a][p] * f[c+b][q],
+// sharing the product f*f of a pair between its D basis functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dgv.h"
+
+namespace dgv {
+
+constexpr int kMaxS = 8;
+
+struct Grid {
+  int Mu = 0, Mv = 0, Mw = 0, su = 0, sv = 0, sw = 0, D = 0;
+  double box[6] = {0, 0, 0, 0, 0, 0};
+  int64_t N = 0, ncells = 0;
+  int ccell = 0;  // 0-based canonical cell
+  int ccu = 0, ccv = 0, ccw = 0;
+  double gn[3][kMaxS] = {}, gw[3][kMaxS] = {};  // Gauss nodes/weights per dimension
+  std::vector<double> u, v, w, gwts;            // node coordinates and weights
+  std::vector<double> lu, ru, lv, rv, lw, rw;   // cell bounds
+};
+
+// Block header, read with scalar loads by the evaluator (uniform per wave).
+struct alignas(16) BlockHdr {
+  int64_t pbeg;        // first pair of the block
+  int8_t a[3], b[3];   // cell offsets of xi and xi1 from the canonical cell
+  int16_t npairs;
+  uint16_t seg[65];    // seg[p] relative to pbeg, p = 0..D (D <= 64)
+};
+
+struct Group {
+  int32_t lo[3], ext[3];  // box of output cells
+  int32_t ncells;
+  int32_t bbeg, bend;     // range in the group's block-index list
+  int64_t npairs;
+};
+
+struct WorkItem {
+  int32_t group, chunk;   // 64-cell chunk of the group's box
+  int32_t bbeg, bend;     // blocks (indices into group_blocks)
+  int64_t prow;           // first partial row (64 rows per item)
+};
+
+// Host-side description of the pair-block layout (built once per A-array).
+struct PairBlock {
+  int D = 0;
+  int64_t npairs = 0;
+  std::vector<BlockHdr> blocks;       // sorted by group
+  std::vector<Group> groups;
+  std::vector<int32_t> group_blocks;  // block indices grouped by box
+  // device copies
+  double* d_pairs = nullptr;          // npairs * (D+1) doubles: T[0..D-1], q bits
+  BlockHdr* d_blocks = nullptr;
+  int32_t* d_group_blocks = nullptr;
+  Group* d_groups = nullptr;
+  int64_t nrec_ref = 0;               // record count in the reference layout
+  std::vector<int64_t> nrec_basis;    // per canonical basis function
+};
+
+// Work decomposition of one evaluation (depends on tile size).
+struct Plan {
+  int32_t pairs_per_item = 0;
+  std::vector<WorkItem> items;
+  int64_t nrows = 0;                  // partial rows = 64 * items
+  WorkItem* d_items = nullptr;
+  double* d_partial = nullptr;        // nrows * D doubles (FP64 partial sums)
+  int64_t* d_cell_ptr = nullptr;      // CSR over output cells of partial rows
+  int32_t* d_cell_rows = nullptr;
+  int64_t ncell_rows = 0;
+};
+
+}  // namespace dgv
+
+struct dgv_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  dgv::Grid g;
+  // device geometry
+  double* d_gwts = nullptr;
+  double* d_u = nullptr;
+  double* d_v = nullptr;
+  double* d_w = nullptr;
+  // A-array
+  bool has_a = false;
+  dgv::PairBlock pb;
+  dgv::Plan plan;
+  int32_t tile = 0;          // pairs per work item (0 = default)
+  int kernel = DGV_KERNEL_AUTO;
+  // reference layout on the device for the record kernel (built lazily)
+  bool has_rec = false;
+  int64_t* d_rec_off = nullptr;   // D+1 offsets of the canonical basis functions
+  double* d_rec_A = nullptr;
+  int32_t* d_rec_xi = nullptr;    // 0-based node ids
+  int32_t* d_rec_xi1 = nullptr;
+  // scratch
+  double* d_f = nullptr;
+  double* d_q = nullptr;
+  double* d_tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* d_red = nullptr;        // moment reduction scratch
+  hipGraphExec_t rk_graph = nullptr;
+  int rk_graph_key[2] = {0, 0};
+  double rk_graph_dt = 0, rk_graph_coef = 0;
+  double* rk_graph_f = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // host copy of the reference layout, built on first export
+  bool exp_valid = false;
+  std::vector<std::vector<std::array<int64_t, 2>>> exp_idx;
+  std::vector<std::vector<double>> exp_val;
+  std::string err;
+};
+
+namespace dgv {
+
+int set_error(dgv_ctx* ctx, int code, const std::string& msg);
+void set_global_error(const std::string& msg);
+
+#define DGV_HIP(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::dgv::set_error((ctx), DGV_ERR_HIP,                                 \
+                              std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// geometry (dgv_host.cpp)
+int build_grid(const dgv_grid_desc& d, Grid& g, std::string& err);
+int gauss_legendre(int n, double* x, double* w);
+
+// layout (dgv_host.cpp)
+int build_pairblock_from_records(dgv_ctx* ctx, const int32_t* capphi, int64_t ncap,
+                                 const double* A, const int32_t* xi, const int32_t* xi1,
+                                 int64_t nrec, int index_base);
+int upload_pairblock(dgv_ctx* ctx, std::vector<double>& pairs);
+int build_plan(dgv_ctx* ctx, int32_t pairs_per_item);
+void free_pairblock(dgv_ctx* ctx);
+void free_plan(dgv_ctx* ctx);
+int ensure_record_layout(dgv_ctx* ctx);
+
+// kernels (dgv_kernels.hip)
+int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
+int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q);
+int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
+int launch_axpy(dgv_ctx* ctx, int64_t n, double* out, const double* x0, int nterms,
+                const double* const* xs, const double* coefs);
+int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);
+int device_moments(dgv_ctx* ctx, const double* d_f, double out[5]);
+int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]);
+int device_project(dgv_ctx* ctx, double* d_q);
+
+// generator (dgv_agen.hip)
+int generate_pairblock(dgv_ctx* ctx, const dgv_agen_params& p);
+
+}  // namespace dgv

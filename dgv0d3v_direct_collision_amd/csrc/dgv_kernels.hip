@@ -1,0 +1,460 @@
+// dgv_kernels.hip -- gfx950 kernels of the direct DGV collision evaluator (FP64)
+// and the device arithmetic of the spatial operator and the time steppers.
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dgv_internal.h"
+
+namespace dgv {
+
+// ---------------------------------------------------------------------------
+// Pair-block evaluator.  One wave per work item (group chunk x block run).
+// Lane = one output cell of the group's box.  Per block the lane holds
+// f[c+a][0..D-1] in registers and f[c+b][0..D-1] in its own LDS column; per
+// pair it forms g = f[c+a][p] * f[c+b][q] once and accumulates the pair's D
+// coefficients (wave-uniform, read with scalar loads) into D accumulators:
+//   acc[l] += T[l] * g.
+// Bounds tests of collision_mod.f90:196-211 are unnecessary: every cell of a
+// group's box is in range for every block of the group by construction.
+template <int D>
+__global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ items,
+                                                const Group* __restrict__ groups,
+                                                const int32_t* __restrict__ gblocks,
+                                                const BlockHdr* __restrict__ blocks,
+                                                const double* __restrict__ pairs,
+                                                const double* __restrict__ f,
+                                                double* __restrict__ partial, int Mv, int Mw) {
+  __shared__ double fb[D * 64];
+  const WorkItem it = items[blockIdx.x];
+  const Group g = groups[it.group];
+  const int lane = threadIdx.x;
+  const int cl = it.chunk * 64 + lane;
+  const bool active = cl < g.ncells;
+  int r = active ? cl : 0;  // idle lanes shadow the box's first cell (in range)
+  const int cw = g.lo[2] + r % g.ext[2];
+  r /= g.ext[2];
+  const int cv = g.lo[1] + r % g.ext[1];
+  r /= g.ext[1];
+  const int cu = g.lo[0] + r;
+
+  double acc[D];
+#pragma unroll
+  for (int l = 0; l < D; ++l) acc[l] = 0.0;
+
+  for (int bi = it.bbeg; bi < it.bend; ++bi) {
+    const BlockHdr* B = blocks + gblocks[bi];
+    const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
+    const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+    const double* fx = f + (int64_t)xc * D;
+    const double* fy = f + (int64_t)yc * D;
+    double fa[D];
+#pragma unroll
+    for (int p = 0; p < D; ++p) fa[p] = fx[p];
+#pragma unroll
+    for (int q = 0; q < D; ++q) fb[q * 64 + lane] = fy[q];
+    const double* P = pairs + B->pbeg * (D + 1);
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+      const int j0 = B->seg[p], j1 = B->seg[p + 1];
+      const double fap = fa[p];
+      for (int j = j0; j < j1; ++j) {
+        const double* R = P + (int64_t)j * (D + 1);
+        const int q = (int)__double_as_longlong(R[D]);
+        const double gq = fap * fb[q * 64 + lane];
+#pragma unroll
+        for (int l = 0; l < D; ++l) acc[l] = __fma_rn(R[l], gq, acc[l]);
+      }
+    }
+  }
+  if (active) {
+    double* out = partial + (it.prow + lane) * D;
+#pragma unroll
+    for (int l = 0; l < D; ++l) out[l] = acc[l];
+  }
+}
+
+// Final FP64 reduction: Q[c,l] = (2 * sum over the cell's partial rows) / w,
+// rows summed in their listed order (deterministic), then times `scale`
+// (the spatial operator's (d/L)^2 N dt, spat_oper_mod.f90:65-66; 1 for Q).
+__global__ void k_pb_reduce(int D, int64_t ncells, const int64_t* __restrict__ cell_ptr,
+                            const int32_t* __restrict__ cell_rows,
+                            const double* __restrict__ partial, const double* __restrict__ gwts,
+                            double scale, double* __restrict__ q) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * D) return;
+  const int64_t c = t / D;
+  const int l = (int)(t - c * D);
+  const int64_t b = cell_ptr[c], e = cell_ptr[c + 1];
+  double s = 0.0;
+  int64_t k = b;
+  for (; k + 4 <= e; k += 4) {
+    const double v0 = partial[(int64_t)cell_rows[k] * D + l];
+    const double v1 = partial[(int64_t)cell_rows[k + 1] * D + l];
+    const double v2 = partial[(int64_t)cell_rows[k + 2] * D + l];
+    const double v3 = partial[(int64_t)cell_rows[k + 3] * D + l];
+    s = s + v0;
+    s = s + v1;
+    s = s + v2;
+    s = s + v3;
+  }
+  for (; k < e; ++k) s = s + partial[(int64_t)cell_rows[k] * D + l];
+  double v = (2.0 * s) / gwts[t];
+  if (scale != 1.0) v = v * scale;
+  q[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Record kernel: the reference's loop structure on the device
+// (collision_mod.f90:186-227).  One workgroup per node iphi; the threads stride
+// over the records of the node's canonical basis function, apply the shift and
+// the six bounds tests, accumulate 2*f(xi)*f(xi1)*A, and reduce in a fixed tree.
+__global__ __launch_bounds__(256) void k_rec_eval(int D, int Mu, int Mv, int Mw, int ccu, int ccv,
+                                                  int ccw, const int64_t* __restrict__ off,
+                                                  const double* __restrict__ A,
+                                                  const int32_t* __restrict__ xi,
+                                                  const int32_t* __restrict__ xi1,
+                                                  const double* __restrict__ f,
+                                                  const double* __restrict__ gwts, double scale,
+                                                  double* __restrict__ q) {
+  __shared__ double red[256];
+  const int64_t iphi = blockIdx.x;
+  const int64_t cell = iphi / D;
+  const int l = (int)(iphi - cell * D);
+  const int cw = (int)(cell % Mw), cv = (int)((cell / Mw) % Mv), cu = (int)(cell / ((int64_t)Mw * Mv));
+  const int du = cu - ccu, dv = cv - ccv, dw = cw - ccw;
+  double acc = 0.0;
+  for (int64_t j = off[l] + threadIdx.x; j < off[l + 1]; j += 256) {
+    const int32_t x = xi[j], y = xi1[j];
+    const int32_t xc = x / D, yc = y / D;
+    const int xu = xc / (Mv * Mw) + du, xv = (xc / Mw) % Mv + dv, xw = xc % Mw + dw;
+    if (xu < 0 || xu >= Mu || xv < 0 || xv >= Mv || xw < 0 || xw >= Mw) continue;
+    const int yu = yc / (Mv * Mw) + du, yv = (yc / Mw) % Mv + dv, yw = yc % Mw + dw;
+    if (yu < 0 || yu >= Mu || yv < 0 || yv >= Mv || yw < 0 || yw >= Mw) continue;
+    const int64_t kx = ((int64_t)(xu * Mv + xv) * Mw + xw) * D + (x - xc * D);
+    const int64_t ky = ((int64_t)(yu * Mv + yv) * Mw + yw) * D + (y - yc * D);
+    acc = acc + 2.0 * f[kx] * f[ky] * A[j];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v = red[0] / gwts[iphi];
+    if (scale != 1.0) v = v * scale;
+    q[iphi] = v;
+  }
+}
+
+int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
+  const Grid& g = ctx->g;
+  const PairBlock& pb = ctx->pb;
+  const Plan& P = ctx->plan;
+  const int nitems = (int)P.items.size();
+  if (nitems > 0) {
+    switch (g.D) {
+#define DGV_PB_CASE(DD)                                                                          \
+  case DD:                                                                                       \
+    hipLaunchKernelGGL(k_pb_eval<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
+                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
+                       g.Mv, g.Mw);                                                             \
+    break;
+      DGV_PB_CASE(1)
+      DGV_PB_CASE(8)
+      DGV_PB_CASE(27)
+#undef DGV_PB_CASE
+      default:
+        return set_error(ctx, DGV_ERR_UNSUPPORTED, "pair-block kernel supports 1, 8 or 27 nodes per cell");
+    }
+  }
+  const int64_t nt = g.ncells * g.D;
+  hipLaunchKernelGGL(k_pb_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream, g.D, g.ncells,
+                     P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
+  int rc = ensure_record_layout(ctx);
+  if (rc) return rc;
+  const Grid& g = ctx->g;
+  hipLaunchKernelGGL(k_rec_eval, dim3((unsigned)g.N), dim3(256), 0, ctx->stream, g.D, g.Mu, g.Mv, g.Mw, g.ccu,
+                     g.ccv, g.ccw, ctx->d_rec_off, ctx->d_rec_A, ctx->d_rec_xi, ctx->d_rec_xi1, d_f, ctx->d_gwts,
+                     scale, d_q);
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+// The dominant kernel alone (no final reduction), for timing.
+int launch_eval_main_only(dgv_ctx* ctx, const double* d_f) {
+  int k = ctx->kernel;
+  const Grid& g = ctx->g;
+  if (k == DGV_KERNEL_AUTO) k = (g.D == 1 || g.D == 8 || g.D == 27) ? DGV_KERNEL_PAIRBLOCK : DGV_KERNEL_RECORD;
+  if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, ctx->d_q, 1.0);
+  const PairBlock& pb = ctx->pb;
+  const Plan& P = ctx->plan;
+  const int nitems = (int)P.items.size();
+  if (nitems == 0) return DGV_OK;
+  switch (g.D) {
+#define DGV_PB_CASE(DD)                                                                          \
+  case DD:                                                                                       \
+    hipLaunchKernelGGL(k_pb_eval<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
+                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
+                       g.Mv, g.Mw);                                                             \
+    break;
+    DGV_PB_CASE(1)
+    DGV_PB_CASE(8)
+    DGV_PB_CASE(27)
+#undef DGV_PB_CASE
+    default:
+      return set_error(ctx, DGV_ERR_UNSUPPORTED, "pair-block kernel supports 1, 8 or 27 nodes per cell");
+  }
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+static bool pairblock_supported(int D) { return D == 1 || D == 8 || D == 27; }
+
+int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
+  if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set (dgv_ctx_set_aarray / generate)");
+  int k = ctx->kernel;
+  if (k == DGV_KERNEL_AUTO) k = pairblock_supported(ctx->g.D) ? DGV_KERNEL_PAIRBLOCK : DGV_KERNEL_RECORD;
+  if (k == DGV_KERNEL_PAIRBLOCK) return launch_eval_pairblock(ctx, d_f, d_q, scale);
+  return launch_eval_record(ctx, d_f, d_q, scale);
+}
+
+// Nonzero coefficients per block (records of the reference layout).
+__global__ void k_block_nnz(int D, int64_t nblocks, const BlockHdr* __restrict__ blocks,
+                            const double* __restrict__ pairs, int64_t* __restrict__ out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const BlockHdr& B = blocks[b];
+  int64_t n = 0;
+  for (int j = 0; j < B.npairs; ++j)
+    for (int l = 0; l < D; ++l) n += pairs[(B.pbeg + j) * (D + 1) + l] != 0.0;
+  out[b] = n;
+}
+
+int workload(dgv_ctx* ctx, int64_t out[4]) {
+  const Grid& g = ctx->g;
+  const PairBlock& pb = ctx->pb;
+  const int64_t nb = (int64_t)pb.blocks.size();
+  std::vector<int64_t> nnz(nb);
+  int64_t* d = nullptr;
+  DGV_HIP(ctx, hipMalloc(&d, sizeof(int64_t) * std::max<int64_t>(nb, 1)));
+  hipLaunchKernelGGL(k_block_nnz, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, g.D, nb,
+                     pb.d_blocks, pb.d_pairs, d);
+  hipError_t e = hipMemcpyAsync(nnz.data(), d, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_error(ctx, DGV_ERR_HIP, std::string("workload: ") + hipGetErrorString(e));
+  const int M[3] = {g.Mu, g.Mv, g.Mw};
+  int64_t R = 0, P = 0, V = 0, PV = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const BlockHdr& B = pb.blocks[b];
+    int64_t box = 1;
+    for (int dd = 0; dd < 3; ++dd) {
+      const int lo = std::max(0, std::max(-(int)B.a[dd], -(int)B.b[dd]));
+      const int hi = std::min(M[dd], std::min(M[dd] - B.a[dd], M[dd] - B.b[dd]));
+      box *= std::max(0, hi - lo);
+    }
+    R += nnz[b];
+    P += B.npairs;
+    V += nnz[b] * box;
+    PV += (int64_t)B.npairs * box;
+  }
+  out[0] = R; out[1] = P; out[2] = V; out[3] = PV;
+  return DGV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Linear combinations of the steppers, evaluated term by term left to right
+// as the reference's Fortran expressions are: out = t0 + t1 + ... with
+// t_k = sign_k * ((x_k * m1_k) * m2_k) / den_k  (time_integr_mod.f90:47-145,
+// 325-357).
+struct LinTerm {
+  const double* x;
+  double m1, m2, den;
+  int neg;
+};
+struct LinComb {
+  LinTerm t[6];
+  int n;
+};
+
+__global__ void k_lincomb(int64_t n, double* __restrict__ out, LinComb c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < c.n; ++k) {
+    const LinTerm& t = c.t[k];
+    double v = t.x[i];
+    if (t.m1 != 1.0) v = v * t.m1;
+    if (t.m2 != 1.0) v = v * t.m2;
+    if (t.den != 1.0) v = v / t.den;
+    if (k == 0) s = t.neg ? -v : v;
+    else s = t.neg ? s - v : s + v;
+  }
+  out[i] = s;
+}
+
+int launch_lincomb(dgv_ctx* ctx, int64_t n, double* out, int nterms, const double* const* xs, const double* m1,
+                   const double* m2, const double* den, const int* neg) {
+  LinComb c;
+  std::memset(&c, 0, sizeof(c));
+  c.n = nterms;
+  for (int k = 0; k < nterms; ++k) {
+    c.t[k].x = xs[k];
+    c.t[k].m1 = m1 ? m1[k] : 1.0;
+    c.t[k].m2 = m2 ? m2[k] : 1.0;
+    c.t[k].den = den ? den[k] : 1.0;
+    c.t[k].neg = neg ? neg[k] : 0;
+  }
+  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n, out, c);
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic FP64 reductions: 256 workgroups x 256 threads, each thread a
+// fixed strided subset, fixed LDS tree, then one workgroup sums the 256
+// partials in order.
+constexpr int kRedBlocks = 256;
+
+// mode 0: {sum f w, sum f w u, sum f w v, sum f w w, 0}
+// mode 1: {0,0,0,0, sum f w |u - ubar|^2} with ubar read from mean[]
+// mode 2: {sum q w psi_k}, psi = 1, u, v, w, |u|^2
+__global__ __launch_bounds__(256) void k_moments_part(int mode, int64_t n, const double* __restrict__ f,
+                                                      const double* __restrict__ w, const double* __restrict__ u,
+                                                      const double* __restrict__ v, const double* __restrict__ ww,
+                                                      const double* __restrict__ mean, double* __restrict__ part) {
+  __shared__ double red[5][256];
+  double s[5] = {0, 0, 0, 0, 0};
+  double ub = 0, vb = 0, wb = 0;
+  if (mode == 1) { ub = mean[1]; vb = mean[2]; wb = mean[3]; }
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kRedBlocks * 256) {
+    const double fw = f[i] * w[i];
+    if (mode == 0) {
+      s[0] += fw;
+      s[1] += fw * u[i];
+      s[2] += fw * v[i];
+      s[3] += fw * ww[i];
+    } else if (mode == 1) {
+      const double du = u[i] - ub, dv = v[i] - vb, dw = ww[i] - wb;
+      s[4] += fw * (du * du + dv * dv + dw * dw);
+    } else {
+      s[0] += fw;
+      s[1] += fw * u[i];
+      s[2] += fw * v[i];
+      s[3] += fw * ww[i];
+      s[4] += fw * (u[i] * u[i] + v[i] * v[i] + ww[i] * ww[i]);
+    }
+  }
+  for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+      for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; ++k) part[blockIdx.x * 5 + k] = red[k][0];
+}
+
+// Sums the partials; mode 0 turns the sums into (n, ubar, vbar, wbar) as
+// MassCheckRec does (dgvtools_mod.f90:1448-1452); mode 1 finishes T (line 1455).
+__global__ void k_moments_final(int mode, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < kRedBlocks; ++b)
+    for (int k = 0; k < 5; ++k) s[k] += part[b * 5 + k];
+  if (mode == 0) {
+    out[0] = s[0];
+    out[1] = s[1] / s[0];
+    out[2] = s[2] / s[0];
+    out[3] = s[3] / s[0];
+  } else if (mode == 1) {
+    out[4] = s[4] / out[0] / 3.0 * 2.0;
+  } else {
+    for (int k = 0; k < 5; ++k) out[k] = s[k];
+  }
+}
+
+int device_moments(dgv_ctx* ctx, const double* d_f, double out[5]) {
+  const Grid& g = ctx->g;
+  double* part = ctx->d_red;
+  double* res = ctx->d_red + kRedBlocks * 5;
+  hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 0, g.N, d_f, ctx->d_gwts,
+                     ctx->d_u, ctx->d_v, ctx->d_w, res, part);
+  hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(1), 0, ctx->stream, 0, part, res);
+  hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 1, g.N, d_f, ctx->d_gwts,
+                     ctx->d_u, ctx->d_v, ctx->d_w, res, part);
+  hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(1), 0, ctx->stream, 1, part, res);
+  DGV_HIP(ctx, hipGetLastError());
+  DGV_HIP(ctx, hipMemcpyAsync(out, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
+int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]) {
+  const Grid& g = ctx->g;
+  double* part = ctx->d_red;
+  double* res = ctx->d_red + kRedBlocks * 5;
+  hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 2, g.N, d_q, ctx->d_gwts,
+                     ctx->d_u, ctx->d_v, ctx->d_w, res, part);
+  hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(1), 0, ctx->stream, 2, part, res);
+  DGV_HIP(ctx, hipGetLastError());
+  DGV_HIP(ctx, hipMemcpyAsync(out, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
+// Projection onto the complement of span{1,u,v,w,|u|^2} in the w-weighted
+// inner product: lambda = G^{-1} b with G_jk = sum w psi_j psi_k (Cholesky
+// factor precomputed on the host in extended precision), q -= sum lambda_k psi_k.
+__global__ void k_project_solve(const double* __restrict__ part, const double* __restrict__ L,
+                                double* __restrict__ lam) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double b[5] = {0, 0, 0, 0, 0};
+  for (int blk = 0; blk < kRedBlocks; ++blk)
+    for (int k = 0; k < 5; ++k) b[k] += part[blk * 5 + k];
+  double y[5];
+  for (int i = 0; i < 5; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[i * 5 + k] * y[k];
+    y[i] = s / L[i * 5 + i];
+  }
+  for (int i = 4; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 5; ++k) s -= L[k * 5 + i] * lam[k];
+    lam[i] = s / L[i * 5 + i];
+  }
+}
+
+__global__ void k_project_apply(int64_t n, double* __restrict__ q, const double* __restrict__ u,
+                                const double* __restrict__ v, const double* __restrict__ w,
+                                const double* __restrict__ lam) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double uu = u[i], vv = v[i], ww = w[i];
+  q[i] = q[i] - (lam[0] + lam[1] * uu + lam[2] * vv + lam[3] * ww + lam[4] * (uu * uu + vv * vv + ww * ww));
+}
+
+int device_project(dgv_ctx* ctx, double* d_q) {
+  const Grid& g = ctx->g;
+  double* part = ctx->d_red;
+  double* lam = ctx->d_red + kRedBlocks * 5 + 8;
+  double* L = ctx->d_red + kRedBlocks * 5 + 16;
+  hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 2, g.N, d_q, ctx->d_gwts,
+                     ctx->d_u, ctx->d_v, ctx->d_w, lam, part);
+  hipLaunchKernelGGL(k_project_solve, dim3(1), dim3(1), 0, ctx->stream, part, L, lam);
+  hipLaunchKernelGGL(k_project_apply, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, ctx->stream, g.N, d_q,
+                     ctx->d_u, ctx->d_v, ctx->d_w, lam);
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+}  // namespace dgv

@@ -1,0 +1,205 @@
+/*
+ * dgv.h -- C ABI of the MI355X-native direct DGV collision evaluator.
+ *
+ * Drop-in replacement for the collision_mod / spat_oper_mod / time_integr_mod
+ * path of alexmalekseenko/DGV0D3V_direct_collision.  Every entry point returns
+ * an int status (DGV_OK = 0, negative = error; text via dgv_last_error) where the
+ * reference prints a message and STOPs.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Node order is the reference's (cell u->v->w outer, local u->v->w inner,
+ *    miscset.f90:915-927,1150-1176); arrays handed in or out use that order.
+ *  - A-arrays handed in use the reference layout (commvar.f90:157-167): per
+ *    node k the A_capphi(k) records of basis function k are contiguous,
+ *    records hold A, A_xi, A_xi1 with node ids in the given index_base.
+ *  - "_dev" entry points take device pointers and enqueue on the context's
+ *    stream without synchronising; the others take host pointers and block.
+ *  - A context is not reentrant: one host thread per context.
+ */
+#ifndef DGV_H
+#define DGV_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DGV_OK = 0,
+  DGV_ERR_ARG = -1,       /* bad argument / size mismatch                    */
+  DGV_ERR_GRID = -2,      /* grid unsupported (origin outside, s > 8, ...)   */
+  DGV_ERR_INDEX = -3,     /* A_xi / A_xi1 out of range for index_base        */
+  DGV_ERR_NOMEM = -4,     /* device or host allocation failed                */
+  DGV_ERR_HIP = -5,       /* HIP runtime error                               */
+  DGV_ERR_STATE = -6,     /* call out of order (no A-array set, ...)         */
+  DGV_ERR_IO = -7,        /* file missing, truncated or corrupt              */
+  DGV_ERR_UNSUPPORTED = -8
+};
+
+enum { DGV_FP64 = 0, DGV_FP32_COMP = 1 };                 /* precision       */
+enum { DGV_KERNEL_AUTO = 0, DGV_KERNEL_PAIRBLOCK = 1, DGV_KERNEL_RECORD = 2 };
+
+typedef struct dgv_ctx dgv_ctx;
+
+/* One uniform velocity grid: SetDGVblzmmesh/Set3DCellsR_DGVblzm inputs
+ * (readwrite.f90:139-347: "left/right endpoint in u", "number of cells in u",
+ * "degree of local Lagrange basis in u" -> su = nodes per cell). */
+typedef struct {
+  int32_t Mu, Mv, Mw;
+  int32_t su, sv, sw;
+  double u_L, u_R, v_L, v_R, w_L, w_R;
+} dgv_grid_desc;
+
+/* Parameters of the A-array generator (SetA_DGV, dgvtools_mod.f90:377-681;
+ * keys of readwrite.f90:652-696). */
+typedef struct {
+  double Trad;        /* cutoff radius ("cutoff radius of A-array")          */
+  double min_sens;    /* drop |A_raw| <= min_sens ("cutoff values for A")     */
+  int32_t nquad;      /* Gauss points per smooth t-interval (0 = default 24)  */
+  int32_t moment_mode;/* 0: Lagrange basis; 1..5: psi = 1,u,v,w,|u|^2 test    */
+} dgv_agen_params;
+
+/* ---- context, geometry ---------------------------------------------------- */
+/* replaces SetDGblzmGnodes + SetDGVblzmmesh + Set3DCellsR_DGVblzm + SetNodesDGV
+ * + SetCellsUGINdsSftArrs(FindCellContainsPoint_DGV(0,0,0))
+ * (DGVblzmPL.f90:75-96, miscmpiset.f90:553) */
+int dgv_ctx_create(const dgv_grid_desc* grid, int device, dgv_ctx** out);
+void dgv_ctx_destroy(dgv_ctx* ctx);
+const char* dgv_last_error(const dgv_ctx* ctx);   /* ctx may be NULL        */
+int dgv_ctx_num_nodes(const dgv_ctx* ctx, int64_t* nnodes);
+/* the same geometry without a context or GPU: node coordinates and weights
+ * (arrays of (Mu*Mv*Mw)*(su*sv*sw) doubles, any may be NULL), 1-based
+ * canonical cell */
+int dgv_grid_geometry(const dgv_grid_desc* grid, double* u, double* v, double* w, double* gwts,
+                      int32_t* ccell, int64_t* nnodes);
+/* 1-based canonical cell, as FindCellContainsPoint_DGV (dgvtools_mod.f90:1175) */
+int dgv_ctx_canonical_cell(const dgv_ctx* ctx, int32_t* ccell);
+/* nodes_u/_v/_w/_gwts (commvar.f90:75-82, SetNodesDGV miscset.f90:1160-1167);
+ * any pointer may be NULL */
+int dgv_ctx_get_nodes(const dgv_ctx* ctx, double* u, double* v, double* w, double* gwts);
+/* nodes_phican / nodes_dui,dvi,dwi (miscset.f90:1298-1305), 1-based phican */
+int dgv_ctx_get_shifts(const dgv_ctx* ctx, int32_t* phican, int32_t* dui, int32_t* dvi,
+                       int32_t* dwi);
+
+/* ---- A-array ----------------------------------------------------------------- */
+/* replaces ReadAarraysChnksOfstNrecDGV's in-memory result (readwrite.f90:1261-1449):
+ * the caller's arrays are copied; records of every node k may be present, only
+ * the canonical cell's are used by the evaluator (collision_mod.f90:187-188). */
+int dgv_ctx_set_aarray(dgv_ctx* ctx, const int32_t* capphi, int64_t ncapphi,
+                       const double* A, const int32_t* A_xi, const int32_t* A_xi1,
+                       int64_t nrec, int index_base);
+/* replaces SetA_DGV for all canonical basis functions (dgvtools_mod.f90:377-681),
+ * run on the GPU; the result becomes the context's A-array. */
+int dgv_ctx_generate_aarray(dgv_ctx* ctx, const dgv_agen_params* params);
+/* Host-side evaluation of one pair (1-based node ids i2 < i3, as SetA_DGV's
+ * loop variables, dgvtools_mod.f90:484-488) with the same arithmetic as the GPU
+ * generator: raw[l] = Atemp_l, A[l] = record value (0 below min_sens); returns
+ * 1 when the pair fails the prefilter.  Needs no GPU. */
+int dgv_agen_pair(const dgv_grid_desc* grid, int64_t i2, int64_t i3, const dgv_agen_params* params,
+                  double* raw, double* A);
+/* record count of the current A-array (reference layout), per canonical basis
+ * function when per_basis != NULL (length su*sv*sw) */
+int dgv_ctx_aarray_size(const dgv_ctx* ctx, int64_t* nrec, int64_t* per_basis);
+/* work of one Q evaluation with the current A-array: out[0] = records R
+ * (reference layout), out[1] = node pairs, out[2] = in-grid record visits V
+ * (the records that pass the bounds tests of collision_mod.f90:196-211, summed
+ * over all nodes), out[3] = in-grid pair visits */
+int dgv_ctx_workload(const dgv_ctx* ctx, int64_t out[4]);
+/* export the records of one canonical basis function (0-based local index l)
+ * in the reference's serial order (xi ascending, then xi1); index_base 0/1 */
+int dgv_ctx_get_aarray_basis(const dgv_ctx* ctx, int32_t l, double* A, int32_t* A_xi,
+                             int32_t* A_xi1, int64_t cap, int64_t* nrec, int index_base);
+/* Fortran-unformatted chunk file of WriteAarraysDGV / ReadAarraysDGV
+ * (readwrite.f90:1066-1118,1459-1484). */
+int dgv_aarray_write_file(const dgv_ctx* ctx, const char* path);
+int dgv_ctx_read_aarray_file(dgv_ctx* ctx, const char* path);
+/* reference chunk-file base name (MakeBaseNameAoperChnksDGV, readwrite.f90:2224-2281) */
+int dgv_aarray_chunk_name(const dgv_grid_desc* grid, const char* dir, const char* base,
+                          int32_t chunk, int32_t k_c, int32_t N_x, char* out, size_t cap);
+
+/* ---- collision operator --------------------------------------------------------- */
+/* EvalCollisionPeriodicAPlus_DGV(f,fc) (collision_mod.f90:143-229):
+ * fc(i) = (2/w_i) sum_r A_r f(xi_r + d_i) f(xi1_r + d_i) over in-grid records. */
+int dgv_eval_q(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
+int dgv_eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_fc, int64_t n);
+/* EvalCollisionPeriodicA_DGV(f,fc) (collision_mod.f90:29-129): same quantity */
+int dgv_eval_q_a(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
+/* selects the kernel used by dgv_eval_q* (DGV_KERNEL_*) */
+int dgv_ctx_set_kernel(dgv_ctx* ctx, int kernel);
+/* tunables of the pair-block kernel (pairs per work item); 0 = keep */
+int dgv_ctx_set_tile(dgv_ctx* ctx, int32_t pairs_per_item);
+int dgv_ctx_autotune(dgv_ctx* ctx, int32_t* best_pairs_per_item, double* best_ms);
+
+/* SpatialOperSH_DGV(f,fcol,dt), run_mode 0 (spat_oper_mod.f90:28-68):
+ * fcol = Q(f) * coef * dt with coef = (mol_diam/L_inf)^2 * N_inf. */
+int dgv_spatial_oper(dgv_ctx* ctx, const double* f, double* fcol, int64_t n, double dt,
+                     double coef);
+int dgv_spatial_oper_dev(dgv_ctx* ctx, const double* d_f, double* d_fcol, int64_t n,
+                         double dt, double coef);
+/* TimeIntegratorORK_SH_DGV(f,dt), rkmts 1..5 (time_integr_mod.f90:26-152);
+ * frhs1 (may be NULL) receives the first stage / dt (line 151). */
+int dgv_rk_step(dgv_ctx* ctx, double* f, int64_t n, double dt, double coef, int32_t rkmts,
+                double* frhs1);
+int dgv_rk_step_dev(dgv_ctx* ctx, double* d_f, int64_t n, double dt, double coef,
+                    int32_t rkmts, double* d_frhs1);
+/* nsteps RK steps replayed from one captured hipGraph of a step */
+int dgv_rk_run_dev(dgv_ctx* ctx, double* d_f, int64_t n, double dt, double coef,
+                   int32_t rkmts, int64_t nsteps, int use_graph);
+/* TimeIntegratorMTS_SH_DGV(f,dt), Adams-Bashforth 1..5 (time_integr_mod.f90:314-364);
+ * hist = f1..f4, frhs1..frhs5 (9*n doubles), updated in place. */
+int dgv_ab_step(dgv_ctx* ctx, double* f, int64_t n, double dt, double coef, int32_t rkmts,
+                double* hist);
+
+/* MassCheckRec (dgvtools_mod.f90:1437-1458): out = {n, ubar, vbar, wbar, T} */
+int dgv_moments(dgv_ctx* ctx, const double* f, int64_t n, double out[5]);
+/* sum_i w_i psi_k(v_i) q_i for psi = 1,u,v,w,|u|^2 (RecordMomDerivRelaxTimes_DGV's
+ * conservation check, dgvtools_mod.f90:1680-1700), FP64 on the device */
+int dgv_collision_moments(dgv_ctx* ctx, const double* q, int64_t n, double out[5]);
+/* FP64 projection of q onto the complement of the collision invariants in the
+ * w-weighted inner product: afterwards sum_i w_i psi_k q_i = 0 for all k.
+ * No counterpart in the reference (its conservation is only as good as A). */
+int dgv_project_invariants(dgv_ctx* ctx, double* q, int64_t n);
+int dgv_project_invariants_dev(dgv_ctx* ctx, double* d_q, int64_t n);
+
+/* ---- batched evaluation: many VDFs, one or several grids ----------------------- */
+/* No counterpart in the reference (it evaluates one VDF per MPI job).  An
+ * instance is one VDF on one of the given grid contexts (each with its A-array).
+ * f of all instances is packed per grid in the [cell][local][instance] order. */
+typedef struct dgv_batch dgv_batch;
+int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_of_instance,
+                     int32_t ninstances, int32_t precision, dgv_batch** out);
+void dgv_batch_destroy(dgv_batch* b);
+/* offset (in doubles) of instance i's first value inside the packed buffers and
+ * the stride between its consecutive nodes */
+int dgv_batch_layout(const dgv_batch* b, int32_t instance, int64_t* offset, int64_t* stride,
+                     int64_t* total);
+/* pack / unpack one instance (host arrays in reference node order) */
+int dgv_batch_set_f(dgv_batch* b, int32_t instance, const double* f, int64_t n);
+int dgv_batch_get_q(dgv_batch* b, int32_t instance, double* q, int64_t n);
+/* Q of every instance, one persistent launch over the (instance, cell) queue */
+int dgv_batch_eval_q(dgv_batch* b);
+/* one RK step (rkmts 1..3) of every instance, optionally hipGraph-replayed */
+int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_t nsteps,
+                     int use_graph);
+int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms);
+/* device pointers of the packed f and q buffers */
+int dgv_batch_device_buffers(dgv_batch* b, double** d_f, double** d_q);
+
+/* ---- stream / timing helpers ----------------------------------------------------- */
+int dgv_ctx_synchronize(dgv_ctx* ctx);
+/* times niter calls of dgv_eval_q_dev on the context's stream with HIP events;
+ * returns the mean milliseconds per call and the mean of the dominant kernel */
+int dgv_time_eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_fc, int32_t niter,
+                        double* ms_per_call, double* ms_main_kernel);
+int dgv_batch_time_eval(dgv_batch* b, int32_t niter, double* ms_per_call,
+                        double* ms_main_kernel);
+/* device memory helpers for callers without a runtime of their own */
+int dgv_device_alloc(dgv_ctx* ctx, size_t bytes, void** dptr);
+int dgv_device_free(dgv_ctx* ctx, void* dptr);
+int dgv_memcpy_htod(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int dgv_memcpy_dtoh(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

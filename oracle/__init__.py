@@ -1,0 +1,319 @@
+"""CPU oracle for the direct DGV collision path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it; the product package
+(``dgv0d3v_direct_collision_amd``) never does.  It wraps ``libdgv_oracle.so``, a
+plain-C restatement of the reference's routines (alexmalekseenko/
+DGV0D3V_direct_collision):
+
+=====================  ==============================================================
+function               reference routine restated
+=====================  ==============================================================
+``Geometry``           SetDGblzmGnodes/GauLeg, OldCreateUmesh, FillCellsArrsDGV,
+                       SetNodesDGV, FindCellContainsPoint_DGV, SetCellsUGINdsSftArrs
+                       (gaussian_mod.f90:4-53, miscset.f90:29-1311)
+``eval_aplus``         EvalCollisionPeriodicAPlus_DGV (collision_mod.f90:143-229)
+``eval_a``             EvalCollisionPeriodicA_DGV (collision_mod.f90:29-129)
+``eval_aplus_comp``    the same loop with Neumaier-compensated accumulation
+``iphi``               IphiDGV (dgvtools_mod.f90:847-877)
+``masscheck``          MassCheckRec (dgvtools_mod.f90:1437-1458)
+``maxwell``            maxwelveldist (distributions_mod.f90:83-94)
+``rk_step``            SpatialOperSH_DGV run_mode 0 + TimeIntegratorORK_SH_DGV
+                       (spat_oper_mod.f90:28-68, time_integr_mod.f90:26-152)
+``ab_step``            TimeIntegratorMTS_SH_DGV (time_integr_mod.f90:314-364)
+``seta`` / ``seta_raw``  SetA_DGV with the Lagrange basis restored, its adaptive
+                       Simpson integration in chi and epsilon (dgvtools_mod.f90:377-681,
+                       938-1122, 232-289)
+=====================  ==============================================================
+
+Index conventions are the reference's: node ids inside ``A_xi``/``A_xi1`` and cell
+ids are 1-based.
+
+Parity pins (what fixes each function, since the reference ships no tests or
+golden vectors) live in ``tests/test_oracle_*.py``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libdgv_oracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class _Grid(C.Structure):
+    _fields_ = [("Mu", C.c_int32), ("Mv", C.c_int32), ("Mw", C.c_int32),
+                ("su", C.c_int32), ("sv", C.c_int32), ("sw", C.c_int32),
+                ("u_L", C.c_double), ("u_R", C.c_double), ("v_L", C.c_double),
+                ("v_R", C.c_double), ("w_L", C.c_double), ("w_R", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class _Geom(C.Structure):
+    _fields_ = [("g", _Grid), ("nnodes", C.c_int64), ("ncells", C.c_int64),
+                ("ccell", C.c_int32),
+                ("cells_lu", _dp), ("cells_ru", _dp), ("cells_lv", _dp), ("cells_rv", _dp),
+                ("cells_lw", _dp), ("cells_rw", _dp),
+                ("cells_ugi", _ip), ("cells_vgi", _ip), ("cells_wgi", _ip),
+                ("nodes_u", _dp), ("nodes_v", _dp), ("nodes_w", _dp), ("nodes_gwts", _dp),
+                ("nodes_pcell", _ip), ("nodes_ui", _ip), ("nodes_vi", _ip), ("nodes_wi", _ip),
+                ("nodes_dui", _ip), ("nodes_dvi", _ip), ("nodes_dwi", _ip),
+                ("nodes_phican", _ip),
+                ("gn_u", C.c_double * 16), ("gn_v", C.c_double * 16), ("gn_w", C.c_double * 16),
+                ("gw_u", C.c_double * 16), ("gw_v", C.c_double * 16), ("gw_w", C.c_double * 16)]
+
+
+_QFN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def build(force: bool = False) -> str:
+    """Compile libdgv_oracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_gauleg.argtypes = [C.c_int, C.c_double, C.c_double, _f64p, _f64p]
+        L.orc_geom_build.argtypes = [C.POINTER(_Grid), C.POINTER(C.POINTER(_Geom))]
+        L.orc_geom_free.argtypes = [C.POINTER(_Geom)]
+        L.orc_ashift.argtypes = [C.POINTER(_Geom), _i32p, _i64p]
+        ev = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p, _f64p, C.c_int64,
+              C.c_int64, C.c_int]
+        L.orc_eval_aplus.argtypes = ev
+        L.orc_eval_a.argtypes = ev
+        L.orc_eval_aplus_list.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p,
+                                          _f64p, _i64p, C.c_int64, C.c_int]
+        L.orc_eval_aplus_comp.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p,
+                                          _f64p, C.c_void_p, C.c_int64, C.c_int64, C.c_int]
+        L.orc_iphi.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p, _f64p, _f64p,
+                               C.c_double, _f64p]
+        L.orc_masscheck.argtypes = [C.POINTER(_Geom), _f64p, _f64p]
+        L.orc_maxwell.argtypes = [C.c_double] * 8
+        L.orc_maxwell.restype = C.c_double
+        L.orc_spatial_oper.argtypes = [_QFN, C.c_void_p, C.c_int64, _f64p, _f64p, C.c_double,
+                                       C.c_double]
+        L.orc_rk_step.argtypes = [_QFN, C.c_void_p, C.c_int64, _f64p, C.c_double, C.c_double,
+                                  C.c_int, _f64p]
+        L.orc_ab_step.argtypes = [_QFN, C.c_void_p, C.c_int64, _f64p, C.c_double, C.c_double,
+                                  C.c_int, _f64p]
+        L.orc_basis.argtypes = [C.POINTER(_Geom), C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_int32]
+        L.orc_basis.restype = C.c_double
+        L.orc_prefilter.argtypes = [C.POINTER(_Geom), C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_double]
+        L.orc_seta_raw.argtypes = [C.POINTER(_Geom), C.c_int, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.orc_seta.argtypes = [C.POINTER(_Geom), C.c_int, C.c_int32, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_int64, _f64p, _i32p, _i32p, C.c_int]
+        L.orc_seta.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def gauleg(n: int, x1: float = -1.0, x2: float = 1.0):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().orc_gauleg(n, x1, x2, x, w) != 0:
+        raise RuntimeError("gauleg did not converge")
+    return x, w
+
+
+def _arr(ptr, n, dtype):
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Geometry:
+    """The commvar geometry arrays of one uniform grid (1-based ids kept)."""
+
+    def __init__(self, M, s=3, box=(-1.0, 1.0)):
+        if np.isscalar(M):
+            M = (M, M, M)
+        if np.isscalar(s):
+            s = (s, s, s)
+        if len(box) == 2:
+            box = (box[0], box[1]) * 3
+        self.grid = _Grid(int(M[0]), int(M[1]), int(M[2]), int(s[0]), int(s[1]), int(s[2]),
+                          *[float(b) for b in box])
+        self._p = C.POINTER(_Geom)()
+        rc = lib().orc_geom_build(C.byref(self.grid), C.byref(self._p))
+        if rc != 0:
+            raise ValueError(f"orc_geom_build failed ({rc})")
+        g = self._p.contents
+        self.nnodes, self.ncells, self.ccell = g.nnodes, g.ncells, g.ccell
+        nn, nc = self.nnodes, self.ncells
+        for name in ("nodes_u", "nodes_v", "nodes_w", "nodes_gwts"):
+            setattr(self, name, _arr(getattr(g, name), nn, np.float64))
+        for name in ("nodes_pcell", "nodes_ui", "nodes_vi", "nodes_wi", "nodes_dui",
+                     "nodes_dvi", "nodes_dwi", "nodes_phican"):
+            setattr(self, name, _arr(getattr(g, name), nn, np.int32))
+        for name in ("cells_lu", "cells_ru", "cells_lv", "cells_rv", "cells_lw", "cells_rw"):
+            setattr(self, name, _arr(getattr(g, name), nc, np.float64))
+        for name in ("cells_ugi", "cells_vgi", "cells_wgi"):
+            setattr(self, name, _arr(getattr(g, name), nc, np.int32))
+        self.M = tuple(M)
+        self.s = tuple(s)
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.orc_geom_free(self._p)
+            self._p = None
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def ashift(self, capphi):
+        out = np.zeros(self.nnodes, np.int64)
+        lib().orc_ashift(self._p, np.ascontiguousarray(capphi, np.int32), out)
+        return out
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def eval_aplus(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes=None, nthreads=1):
+    fc = np.zeros(G.nnodes)
+    b, e = (0, G.nnodes) if nodes is None else nodes
+    lib().orc_eval_aplus(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                         _c(A_xi1, np.int32), _c(f, np.float64), fc, b, e, nthreads)
+    return fc
+
+
+def eval_aplus_list(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes, nthreads=1):
+    """a1 at the given 0-based nodes only; returns Q at those nodes."""
+    nodes = _c(nodes, np.int64)
+    fc = np.zeros(len(nodes))
+    lib().orc_eval_aplus_list(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                              _c(A_xi1, np.int32), _c(f, np.float64), fc, nodes, len(nodes),
+                              nthreads)
+    return fc
+
+
+def eval_a(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes=None, nthreads=1):
+    fc = np.zeros(G.nnodes)
+    b, e = (0, G.nnodes) if nodes is None else nodes
+    rc = lib().orc_eval_a(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                          _c(A_xi1, np.int32), _c(f, np.float64), fc, b, e, nthreads)
+    if rc:
+        raise MemoryError("orc_eval_a")
+    return fc
+
+
+def eval_aplus_comp(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes=None, nthreads=1):
+    """Returns (Q, sum|terms|/w) with a compensated accumulator."""
+    fc = np.zeros(G.nnodes)
+    ab = np.zeros(G.nnodes)
+    b, e = (0, G.nnodes) if nodes is None else nodes
+    lib().orc_eval_aplus_comp(G.ptr, _c(capphi, np.int32), _c(A, np.float64),
+                              _c(A_xi, np.int32), _c(A_xi1, np.int32), _c(f, np.float64), fc,
+                              ab.ctypes.data, b, e, nthreads)
+    return fc, ab
+
+
+def iphi(capphi, A, A_xi, A_xi1, gwts, f, coef=1.0):
+    n = len(capphi)
+    out = np.zeros(n)
+    lib().orc_iphi(n, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                   _c(A_xi1, np.int32), _c(gwts, np.float64), _c(f, np.float64), coef, out)
+    return out
+
+
+def masscheck(G: Geometry, f):
+    out = np.zeros(5)
+    lib().orc_masscheck(G.ptr, _c(f, np.float64), out)
+    return out
+
+
+def maxwell(T, u0, v0, w0, n, u, v, w):
+    return lib().orc_maxwell(T, u0, v0, w0, n, u, v, w)
+
+
+def _wrap_q(q, n):
+    def cb(_user, fp, fcp):
+        f = np.ctypeslib.as_array(fp, shape=(n,))
+        fc = np.ctypeslib.as_array(fcp, shape=(n,))
+        fc[:] = q(f.copy())
+        return 0
+    return _QFN(cb)
+
+
+def spatial_oper(q, f, dt, coef=1.0):
+    n = len(f)
+    out = np.zeros(n)
+    cb = _wrap_q(q, n)
+    rc = lib().orc_spatial_oper(cb, None, n, _c(f, np.float64), out, dt, coef)
+    if rc:
+        raise RuntimeError("spatial_oper")
+    return out
+
+
+def rk_step(q, f, dt, coef=1.0, rkmts=2):
+    """One TimeIntegratorORK_SH_DGV step; q(f) -> Q(f).  Returns (f_new, frhs1)."""
+    f = _c(f, np.float64).copy()
+    n = len(f)
+    frhs1 = np.zeros(n)
+    cb = _wrap_q(q, n)
+    rc = lib().orc_rk_step(cb, None, n, f, dt, coef, rkmts, frhs1)
+    if rc:
+        raise RuntimeError(f"rk_step rc={rc}")
+    return f, frhs1
+
+
+def ab_step(q, f, hist, dt, coef=1.0, rkmts=2):
+    """One TimeIntegratorMTS_SH_DGV step; hist is (9, n) = f1..f4, frhs1..frhs5."""
+    f = _c(f, np.float64).copy()
+    n = len(f)
+    h = _c(hist, np.float64).reshape(-1).copy()
+    cb = _wrap_q(q, n)
+    rc = lib().orc_ab_step(cb, None, n, f, dt, coef, rkmts, h)
+    if rc:
+        raise RuntimeError(f"ab_step rc={rc}")
+    return f, h.reshape(9, n)
+
+
+def basis(G: Geometry, mode, u, v, w, i1):
+    return lib().orc_basis(G.ptr, mode, u, v, w, i1)
+
+
+def prefilter(G: Geometry, i1, i2, i3, Trad=100.0):
+    return bool(lib().orc_prefilter(G.ptr, i1, i2, i3, Trad))
+
+
+def seta_raw(G: Geometry, i1, i2, i3, ErrChi, ErrEps, mode=0):
+    out = C.c_double()
+    rc = lib().orc_seta_raw(G.ptr, mode, i1, i2, i3, ErrChi, ErrEps, C.byref(out))
+    if rc:
+        raise RuntimeError("SetA_DGV: number of nodes too big")
+    return out.value
+
+
+def seta(G: Geometry, i1, Trad=100.0, ErrChi=1e-6, ErrEps=1e-6, min_sens=0.0, mode=0,
+         cap=None, nthreads=8):
+    """SetA_DGV for one basis function: records (A, A_xi, A_xi1) in the reference's order."""
+    if cap is None:
+        cap = G.nnodes * (G.nnodes - 1) // 2
+    A = np.zeros(cap)
+    xi = np.zeros(cap, np.int32)
+    xi1 = np.zeros(cap, np.int32)
+    n = lib().orc_seta(G.ptr, mode, i1, Trad, ErrChi, ErrEps, min_sens, cap, A, xi, xi1,
+                       nthreads)
+    if n < 0:
+        raise RuntimeError(f"orc_seta failed ({n})")
+    return A[:n].copy(), xi[:n].copy(), xi1[:n].copy()
