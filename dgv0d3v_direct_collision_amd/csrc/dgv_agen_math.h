@@ -14,16 +14,16 @@
 // sin chi d chi d eps is the surface measure of the unit sphere, so the gain
 // term equals 2 * S_l with S_l = int_{S^2} phi_l(c + r k) dk, and the loss term
 // is 4 pi (phi_l(xi) + phi_l(xi1)).  S_l is computed with the polar axis along w:
-//   S_l = int dt L_k(w^(t)) int_{arcs(t)} L_i(u^(eps)) L_j(v^(eps)) d eps,
-// t = cos(theta), rho = r sqrt(1-t^2), u = c_u + rho cos eps, v = c_v + rho sin eps.
+//   S_l = int sin(th) d th L_k(w^(th)) int_{arcs(th)} L_i(u^(eps)) L_j(v^(eps)) d eps,
+// w = c_w + r cos(th), rho = r sin(th), u = c_u + rho cos eps, v = c_v + rho sin eps.
 //  * The inner integral runs over the arcs of the circle of radius rho that lie
 //    in [u0,u1]x[v0,v1]; the integrand is a polynomial in (cos eps, sin eps), so
 //    it is integrated exactly with antiderivatives of cos^k sin^n.
-//  * The outer integral is split at every t where the arc structure changes
-//    (circle tangent to a face line, circle through a corner, the w-faces, the
-//    poles).  Inside a piece the integrand is analytic; at its ends it has at
-//    worst square-root behaviour, which the substitution
-//    t = ta + (tb-ta)(1-cos s)/2 removes, so Gauss-Legendre in s converges
+//  * The outer integral is split at every polar angle where the arc structure
+//    changes (circle tangent to a face line, circle through a corner, the
+//    w-faces).  Inside a piece the integrand is analytic in th; at its ends it
+//    has at worst square-root behaviour, which the substitution
+//    th = ta + (tb-ta)(1-cos s)/2 removes, so Gauss-Legendre in s converges
 //    exponentially.
 #pragma once
 #include <cmath>
@@ -184,29 +184,32 @@ DGV_HD inline void shifted_coeffs(const double mono[kAgenMaxS][kAgenMaxS], doubl
 template <int su, int sv, int sw>
 DGV_HD inline void sphere_integrals(const AgenConsts& C, const double cen[3], double r, double* S) {
   constexpr int D = su * sv * sw;
+  const double pi = 3.141592653589793238462643383279502884;
   for (int l = 0; l < D; ++l) S[l] = 0.0;
-  double tlo = (C.w0 - cen[2]) / r, thi = (C.w1 - cen[2]) / r;
-  if (tlo < -1.0) tlo = -1.0;
-  if (thi > 1.0) thi = 1.0;
-  if (!(thi > tlo)) return;
-  // breakpoints: tangencies to the four face lines and passages through the
-  // four corners of the cell's u-v rectangle
+  // polar angle theta about the w axis: w = c_w + r cos(theta), rho = r sin(theta)
+  // is analytic in theta (no branch points at the poles)
+  const double ca = (C.w1 - cen[2]) / r, cb = (C.w0 - cen[2]) / r;
+  const double th_lo = ca >= 1.0 ? 0.0 : (ca <= -1.0 ? pi : acos(ca));
+  const double th_hi = cb >= 1.0 ? 0.0 : (cb <= -1.0 ? pi : acos(cb));
+  if (!(th_hi > th_lo)) return;
+  // breakpoints: tangency to the four face lines and passage through the four
+  // corners of the cell's u-v rectangle, at rho = d, i.e. theta = asin(d/r)
+  // and pi - asin(d/r)
   double bp[20];
   int nb = 0;
-  bp[nb++] = tlo;
-  bp[nb++] = thi;
+  bp[nb++] = th_lo;
+  bp[nb++] = th_hi;
   const double du[2] = {C.u0 - cen[0], C.u1 - cen[0]}, dv[2] = {C.v0 - cen[1], C.v1 - cen[1]};
-  const double r2 = r * r;
-  auto add = [&](double d2) {
-    if (d2 < r2) {
-      const double t = sqrt(1.0 - d2 / r2);
-      if (t > tlo && t < thi) bp[nb++] = t;
-      if (-t > tlo && -t < thi) bp[nb++] = -t;
+  auto add = [&](double d) {
+    if (d < r) {
+      const double t = asin(d / r);
+      if (t > th_lo && t < th_hi) bp[nb++] = t;
+      if (pi - t > th_lo && pi - t < th_hi) bp[nb++] = pi - t;
     }
   };
-  for (int e = 0; e < 2; ++e) { add(du[e] * du[e]); add(dv[e] * dv[e]); }
+  for (int e = 0; e < 2; ++e) { add(fabs(du[e])); add(fabs(dv[e])); }
   for (int e = 0; e < 2; ++e)
-    for (int f = 0; f < 2; ++f) add(du[e] * du[e] + dv[f] * dv[f]);
+    for (int f = 0; f < 2; ++f) add(sqrt(du[e] * du[e] + dv[f] * dv[f]));
   sort_small(bp, nb);
   const double hu = 0.5 * (C.u1 - C.u0), hv = 0.5 * (C.v1 - C.v0), hw = 0.5 * (C.w1 - C.w0);
   const double mu = 0.5 * (C.u1 + C.u0), mv = 0.5 * (C.v1 + C.v0), mw = 0.5 * (C.w1 + C.w0);
@@ -218,14 +221,15 @@ DGV_HD inline void sphere_integrals(const AgenConsts& C, const double cen[3], do
     const double half = 0.5 * (tb - ta);
     for (int q = 0; q < C.nq; ++q) {
       const double sq = C.qs[q];
-      const double t = ta + half * (1.0 - cos(sq));
-      const double wt = C.qw[q] * half * sin(sq);
-      const double rho = r * sqrt((1.0 - t) * (1.0 + t));
+      const double th = ta + half * (1.0 - cos(sq));
+      const double sth = sin(th);
+      const double wt = C.qw[q] * half * sin(sq) * sth;   // dt = sin(theta) dtheta
+      const double rho = r * sth;
       if (!(rho > 0.0)) continue;
       arc_moments<su, sv>(C, cen[0], cen[1], rho, I);
       shifted_coeffs<su>(C.mono[0], au, rho / hu, cU);
       shifted_coeffs<sv>(C.mono[1], av, rho / hv, cV);
-      const double wn = (cen[2] + r * t - mw) / hw;
+      const double wn = (cen[2] + r * cos(th) - mw) / hw;
       double Lw[kAgenMaxS];
       for (int k = 0; k < sw; ++k) Lw[k] = wt * lagr(k, wn, C.kap[2], sw);
       for (int i = 0; i < su; ++i)

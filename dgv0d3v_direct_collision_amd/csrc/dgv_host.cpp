@@ -231,9 +231,9 @@ static int finish_pairblock(dgv_ctx* ctx, std::vector<uint64_t>& pkeys, std::vec
       int64_t k = i;
       for (int p = 0; p <= D; ++p) {
         while (k < j && (int)((pkeys[k] >> 6) & 63) < p) ++k;
-        h.seg[p] = (uint16_t)(k - i);
+        h.seg[p] = (int32_t)(k - i);
       }
-      h.seg[D] = (uint16_t)(j - i);
+      h.seg[D] = (int32_t)(j - i);
       blocks.push_back(h);
       bkeys.push_back(bk);
       kept.insert(kept.end(), pairs.begin() + i * (D + 1), pairs.begin() + j * (D + 1));
@@ -404,19 +404,41 @@ int build_plan(dgv_ctx* ctx, int32_t ppi) {
       b0 = b1;
     }
   }
-  // largest items first (the hardware dispatcher fills CUs in order)
-  std::vector<size_t> ord(items.size());
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return item_pairs[x] > item_pairs[y]; });
-  P.items.resize(items.size());
-  for (size_t i = 0; i < ord.size(); ++i) {
-    P.items[i] = items[ord[i]];
-    P.items[i].prow = (int64_t)i * 64;
+  // XCD-aware order.  Workgroup i is dispatched to XCD i % 8, and the chunks
+  // of one (group, block run) read the same pair records, so they are placed
+  // on one XCD (ids x, x+8, x+16, ...) to share its L2.  Runs are dealt to the
+  // least-loaded XCD, largest first; columns are padded with empty slots.
+  constexpr int kXcd = 8;
+  struct Run { size_t first; int n; int64_t work; };
+  std::vector<Run> runs;
+  for (size_t i = 0; i < items.size();) {
+    size_t j = i;
+    while (j < items.size() && items[j].group == items[i].group && items[j].bbeg == items[i].bbeg) ++j;
+    runs.push_back({i, (int)(j - i), item_pairs[i] * (int64_t)(j - i)});
+    i = j;
   }
+  std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) { return x.work > y.work; });
+  std::vector<std::vector<size_t>> col(kXcd);
+  std::vector<int64_t> load(kXcd, 0);
+  for (const Run& r : runs) {
+    int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (int k = 0; k < r.n; ++k) col[c].push_back(r.first + k);
+    load[c] += r.work;
+  }
+  size_t nslot = 0;
+  for (auto& cvec : col) nslot = std::max(nslot, cvec.size());
+  P.items.assign(nslot * kXcd, WorkItem{0, -1, 0, 0, 0});
+  for (size_t rr = 0; rr < nslot; ++rr)
+    for (int c = 0; c < kXcd; ++c) {
+      const size_t pos = rr * kXcd + c;
+      if (rr < col[c].size()) P.items[pos] = items[col[c][rr]];
+      P.items[pos].prow = (int64_t)pos * 64;
+    }
   P.nrows = (int64_t)P.items.size() * 64;
   // per-cell row lists
   std::vector<int64_t> cnt(g.ncells + 1, 0);
   for (const WorkItem& it : P.items) {
+    if (it.chunk < 0) continue;
     const Group& G = pb.groups[it.group];
     const int n = std::min(64, G.ncells - it.chunk * 64);
     for (int l = 0; l < n; ++l) {
@@ -431,6 +453,7 @@ int build_plan(dgv_ctx* ctx, int32_t ppi) {
   std::vector<int32_t> rows(cnt[g.ncells]);
   std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
   for (const WorkItem& it : P.items) {
+    if (it.chunk < 0) continue;
     const Group& G = pb.groups[it.group];
     const int n = std::min(64, G.ncells - it.chunk * 64);
     for (int l = 0; l < n; ++l) {

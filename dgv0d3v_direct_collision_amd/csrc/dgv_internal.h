@@ -49,7 +49,7 @@ struct alignas(16) BlockHdr {
   int64_t pbeg;        // first pair of the block
   int8_t a[3], b[3];   // cell offsets of xi and xi1 from the canonical cell
   int16_t npairs;
-  uint16_t seg[65];    // seg[p] relative to pbeg, p = 0..D (D <= 64)
+  int32_t seg[65];     // seg[p] relative to pbeg, p = 0..D (D <= 64)
 };
 
 struct Group {
@@ -60,7 +60,7 @@ struct Group {
 };
 
 struct WorkItem {
-  int32_t group, chunk;   // 64-cell chunk of the group's box
+  int32_t group, chunk;   // 64-cell chunk of the group's box (chunk < 0: empty slot)
   int32_t bbeg, bend;     // blocks (indices into group_blocks)
   int64_t prow;           // first partial row (64 rows per item)
 };

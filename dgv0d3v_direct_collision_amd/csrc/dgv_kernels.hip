@@ -28,6 +28,7 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
                                                 double* __restrict__ partial, int Mv, int Mw) {
   __shared__ double fb[D * 64];
   const WorkItem it = items[blockIdx.x];
+  if (it.chunk < 0) return;  // padding slot of the XCD-aware order
   const Group g = groups[it.group];
   const int lane = threadIdx.x;
   const int cl = it.chunk * 64 + lane;
@@ -38,6 +39,12 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
   const int cv = g.lo[1] + r % g.ext[1];
   r /= g.ext[1];
   const int cu = g.lo[0] + r;
+  constexpr int64_t kRec = (D + 1) * 8;     // bytes per pair record
+  constexpr int64_t kWin = 64 * 128;        // one prefetch instruction: 64 lanes x 128-B lines
+  // L2 prefetch of the pair records: an ordinary (lane-divergent, so vector)
+  // load of one dword per 128-B line, kept alive by folding it into `sink`
+  // one segment later, when it has long arrived (no wait on the critical path).
+  int sink = 0, pending = 0;
 
   double acc[D];
 #pragma unroll
@@ -47,6 +54,10 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
     const BlockHdr* B = blocks + gblocks[bi];
     const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
     const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+    const double* P = pairs + B->pbeg * (D + 1);
+    const char* Pb = reinterpret_cast<const char*>(P);
+    const int64_t bend = (int64_t)B->seg[D] * kRec;
+    int64_t pf = 0;
     const double* fx = f + (int64_t)xc * D;
     const double* fy = f + (int64_t)yc * D;
     double fa[D];
@@ -54,55 +65,76 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
     for (int p = 0; p < D; ++p) fa[p] = fx[p];
 #pragma unroll
     for (int q = 0; q < D; ++q) fb[q * 64 + lane] = fy[q];
-    const double* P = pairs + B->pbeg * (D + 1);
 #pragma unroll
     for (int p = 0; p < D; ++p) {
       const int j0 = B->seg[p], j1 = B->seg[p + 1];
+      // keep the next 16 KB of records on their way to L2
+      sink ^= pending;
+      if (pf < bend && pf < (int64_t)j1 * kRec + 2 * kWin) {
+        const int64_t o = pf + (int64_t)lane * 128;
+        pending = o < bend ? *reinterpret_cast<const int*>(Pb + o) : 0;
+        pf += kWin;
+      }
       const double fap = fa[p];
+      // q of the next pair is read together with the current coefficients,
+      // so each pair costs one scalar-load round trip
+      int qn = j0 < j1 ? (int)__double_as_longlong(P[(int64_t)j0 * (D + 1) + D]) : 0;
       for (int j = j0; j < j1; ++j) {
         const double* R = P + (int64_t)j * (D + 1);
-        const int q = (int)__double_as_longlong(R[D]);
-        const double gq = fap * fb[q * 64 + lane];
+        const double fbq = fb[qn * 64 + lane];
+        if (j + 1 < j1) qn = (int)__double_as_longlong(R[(D + 1) + D]);
+        const double gq = fap * fbq;
 #pragma unroll
         for (int l = 0; l < D; ++l) acc[l] = __fma_rn(R[l], gq, acc[l]);
       }
     }
   }
+  sink ^= pending;
   if (active) {
     double* out = partial + (it.prow + lane) * D;
 #pragma unroll
     for (int l = 0; l < D; ++l) out[l] = acc[l];
   }
+  if (sink == 0x5eed5eed && it.prow < 0) partial[0] = (double)sink;  // never true; keeps the prefetch
 }
 
-// Final FP64 reduction: Q[c,l] = (2 * sum over the cell's partial rows) / w,
-// rows summed in their listed order (deterministic), then times `scale`
-// (the spatial operator's (d/L)^2 N dt, spat_oper_mod.f90:65-66; 1 for Q).
-__global__ void k_pb_reduce(int D, int64_t ncells, const int64_t* __restrict__ cell_ptr,
-                            const int32_t* __restrict__ cell_rows,
-                            const double* __restrict__ partial, const double* __restrict__ gwts,
-                            double scale, double* __restrict__ q) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncells * D) return;
-  const int64_t c = t / D;
-  const int l = (int)(t - c * D);
+// Final FP64 reduction, one wave per output cell: lane j sums the cell's
+// partial rows j, j+64, ... in order (all D basis functions at once), then a
+// fixed binary tree over the 64 lanes in LDS; Q = (2 * sum) / w, times
+// `scale` (the spatial operator's (d/L)^2 N dt, spat_oper_mod.f90:65-66).
+template <int D>
+__global__ __launch_bounds__(64) void k_pb_reduce(int64_t ncells, const int64_t* __restrict__ cell_ptr,
+                                                  const int32_t* __restrict__ cell_rows,
+                                                  const double* __restrict__ partial,
+                                                  const double* __restrict__ gwts, double scale,
+                                                  double* __restrict__ q) {
+  __shared__ double red[64 * (D + 1)];
+  const int64_t c = blockIdx.x;
+  const int lane = threadIdx.x;
   const int64_t b = cell_ptr[c], e = cell_ptr[c + 1];
-  double s = 0.0;
-  int64_t k = b;
-  for (; k + 4 <= e; k += 4) {
-    const double v0 = partial[(int64_t)cell_rows[k] * D + l];
-    const double v1 = partial[(int64_t)cell_rows[k + 1] * D + l];
-    const double v2 = partial[(int64_t)cell_rows[k + 2] * D + l];
-    const double v3 = partial[(int64_t)cell_rows[k + 3] * D + l];
-    s = s + v0;
-    s = s + v1;
-    s = s + v2;
-    s = s + v3;
+  double s[D];
+#pragma unroll
+  for (int l = 0; l < D; ++l) s[l] = 0.0;
+  for (int64_t k = b + lane; k < e; k += 64) {
+    const double* row = partial + (int64_t)cell_rows[k] * D;
+#pragma unroll
+    for (int l = 0; l < D; ++l) s[l] = s[l] + row[l];
   }
-  for (; k < e; ++k) s = s + partial[(int64_t)cell_rows[k] * D + l];
-  double v = (2.0 * s) / gwts[t];
-  if (scale != 1.0) v = v * scale;
-  q[t] = v;
+#pragma unroll
+  for (int l = 0; l < D; ++l) red[lane * (D + 1) + l] = s[l];
+  __syncthreads();
+  for (int h = 32; h > 0; h >>= 1) {
+    if (lane < h) {
+#pragma unroll
+      for (int l = 0; l < D; ++l) red[lane * (D + 1) + l] = red[lane * (D + 1) + l] + red[(lane + h) * (D + 1) + l];
+    }
+    __syncthreads();
+  }
+  for (int l = lane; l < D; l += 64) {
+    double v = (2.0 * red[l]) / gwts[c * D + l];
+    if (scale != 1.0) v = v * scale;
+    q[c * D + l] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -170,9 +202,17 @@ int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double s
         return set_error(ctx, DGV_ERR_UNSUPPORTED, "pair-block kernel supports 1, 8 or 27 nodes per cell");
     }
   }
-  const int64_t nt = g.ncells * g.D;
-  hipLaunchKernelGGL(k_pb_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream, g.D, g.ncells,
-                     P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);
+  switch (g.D) {
+#define DGV_RED_CASE(DD)                                                                               \
+  case DD:                                                                                             \
+    hipLaunchKernelGGL(k_pb_reduce<DD>, dim3((unsigned)g.ncells), dim3(64), 0, ctx->stream, g.ncells,  \
+                       P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);             \
+    break;
+    DGV_RED_CASE(1)
+    DGV_RED_CASE(8)
+    DGV_RED_CASE(27)
+#undef DGV_RED_CASE
+  }
   DGV_HIP(ctx, hipGetLastError());
   return DGV_OK;
 }
