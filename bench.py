@@ -95,9 +95,23 @@ def main():
     d.htod(d_f, f)
 
     tile = None
+    kernels = {}
     if not args.profile:
+        # the other evaluator variants, for the record
+        for name, kid in (("record", dgv.DGV_KERNEL_RECORD), ("pairblock_valu", dgv.DGV_KERNEL_PAIRBLOCK),
+                          ("pairblock_mfma", dgv.DGV_KERNEL_MFMA)):
+            if name == "record" and M > 5:
+                continue  # the reference-order kernel re-reads every record per node
+            d.set_kernel(kid)
+            d.eval_q_dev(d_f, d_q)
+            ms_c, ms_k = d.time_eval_q_dev(d_f, d_q, 3)
+            kernels[name] = {"ms_per_q": ms_c, "main_kernel_ms": ms_k}
+            log(f"kernel {name}: {ms_c:.3f} ms/Q")
+        best = min(kernels, key=lambda k: kernels[k]["ms_per_q"])
+        d.set_kernel({"record": dgv.DGV_KERNEL_RECORD, "pairblock_valu": dgv.DGV_KERNEL_PAIRBLOCK,
+                      "pairblock_mfma": dgv.DGV_KERNEL_MFMA}[best])
         tile, tms = d.autotune()
-        log(f"autotune: {tile} pairs per work item ({tms:.2f} ms)")
+        log(f"kernel {best}; autotune: {tile} pairs per work item ({tms:.2f} ms)")
     for _ in range(args.warmup):
         d.eval_q_dev(d_f, d_q)
     d.synchronize()
@@ -181,10 +195,13 @@ def main():
                                "(3 Gauss x 9 cells per dim on [-4,4]^3), A-array by the GPU generator",
                    "M": M, "nodes": N, "records": wl["records"], "pairs": wl["pairs"],
                    "visits_per_q": wl["visits"], "pairs_per_item": tile, "agen_seconds": agen_s,
+                   "kernels": kernels,
                    "seed": seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pb_eval<27>", "kernel_ms": ms_main,
+                     "kernel": {"pairblock_mfma": "k_pb_mfma<27>", "pairblock_valu": "k_pb_eval<27>",
+                                "record": "k_rec_eval"}.get(best if kernels else "", "k_pb_mfma<27>"),
+                     "kernel_ms": ms_main,
                      "bytes_per_launch": bytes_q,
                      "valu_fp64_tflops": valu, "valu_frac": valu / FP64_VALU_PEAK_TFLOPS},
         "cpu_baseline": cpu,

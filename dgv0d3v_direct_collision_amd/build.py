@@ -54,7 +54,35 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     subprocess.run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def amdflang():
+    for c in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose: bool = False):
+    """The iso_c_binding shim (fortran/dgv_gpu.f90) and its example driver."""
+    fc = amdflang()
+    if fc is None:
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    obj = os.path.join(bdir, "dgv_gpu.o")
+    exe = os.path.join(HERE, "dgv_driver")
+    cmds = [[fc, "-O2", f"-J{bdir}", "-c", os.path.join(fdir, "dgv_gpu.f90"), "-o", obj],
+            [fc, "-O2", f"-I{bdir}", f"-J{bdir}", os.path.join(fdir, "dgv_driver.f90"), obj,
+             f"-L{HERE}", "-ldgv_amd", "-Wl,-rpath,$ORIGIN", "-o", exe]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return exe
 
 
 if __name__ == "__main__":

@@ -339,3 +339,72 @@ class DGV:
         self._chk(self._L.dgv_time_eval_q_dev(self._ctx, C.c_void_p(d_f), C.c_void_p(d_q), niter,
                                               C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class DGVBatch:
+    """Many VDFs on one or several grids (each a ``DGV`` with its A-array),
+    evaluated together in one persistent launch.  precision: DGV_FP64 or
+    DGV_FP32_COMP (FP32 products with two-sum compensated sums, FP64 final sum)."""
+
+    def __init__(self, grids, grid_of_instance, precision=_lib.DGV_FP64):
+        self._L = load()
+        self.grids = list(grids)
+        self.grid_of_instance = np.ascontiguousarray(grid_of_instance, np.int32)
+        arr = (C.c_void_p * len(self.grids))(*[g.ctx for g in self.grids])
+        p = C.c_void_p()
+        check(self._L.dgv_batch_create(arr, len(self.grids), iptr(self.grid_of_instance),
+                                       len(self.grid_of_instance), precision, C.byref(p)))
+        self._b = p
+        self.ninstances = len(self.grid_of_instance)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._L.dgv_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        check(rc, None)
+
+    def set_f(self, i: int, f):
+        f = np.ascontiguousarray(f, np.float64)
+        self._chk(self._L.dgv_batch_set_f(self._b, i, dptr(f), len(f)))
+
+    def get_q(self, i: int) -> np.ndarray:
+        n = self.grids[self.grid_of_instance[i]].nnodes
+        q = np.zeros(n)
+        self._chk(self._L.dgv_batch_get_q(self._b, i, dptr(q), n))
+        return q
+
+    def get_f(self, i: int) -> np.ndarray:
+        """The instance's current f (after stepping)."""
+        off, stride, total = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(self._L.dgv_batch_layout(self._b, i, C.byref(off), C.byref(stride), C.byref(total)))
+        df, dq = C.c_void_p(), C.c_void_p()
+        self._chk(self._L.dgv_batch_device_buffers(self._b, C.byref(df), C.byref(dq)))
+        g = self.grids[self.grid_of_instance[i]]
+        allf = np.zeros(total.value)
+        g.dtoh(allf, df.value)
+        return allf[off.value::stride.value][:g.nnodes].copy()
+
+    def eval_q(self):
+        self._chk(self._L.dgv_batch_eval_q(self._b))
+
+    def rk_run(self, dt: float, nsteps: int, rkmts: int = 2, coef: float = 1.0, use_graph=True):
+        self._chk(self._L.dgv_batch_rk_run(self._b, dt, coef, rkmts, nsteps, int(use_graph)))
+
+    def autotune(self):
+        best = C.c_int32()
+        ms = C.c_double()
+        self._chk(self._L.dgv_batch_autotune(self._b, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
+    def time_eval(self, niter: int = 3) -> float:
+        a = C.c_double()
+        self._chk(self._L.dgv_batch_time_eval(self._b, niter, C.byref(a), None))
+        return a.value

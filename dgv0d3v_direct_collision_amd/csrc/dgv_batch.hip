@@ -1,20 +1,749 @@
-// dgv_batch.hip -- batched evaluation of many VDFs (placeholder entry points;
-// filled in below as the batched kernels land).
+// dgv_batch.hip -- many VDFs, on one or several grids, in one persistent launch.
+//
+// Every instance is a VDF on one of the given grid contexts (each carrying its
+// own A-array in the pair-block layout).  The values of all instances of a grid
+// are packed node-major, instance-minor: F[node * Bpad + instance] with Bpad the
+// instance count rounded up to 64, so that the f values one block needs for 64
+// instances are one coalesced row.
+//
+// Work decomposition.  For an output cell c the contributing blocks are those
+// whose box contains c; their list is cut into segments of about `tile` pairs.
+// A work item is (grid, cell, segment, chunk of 64 instances), one wave; the
+// items of all grids form one queue consumed by a persistent launch.  For each
+// block of its segment the wave stages f[c+a][*][64 instances] and
+// f[c+b][*][64 instances] in LDS and accumulates, on the matrix cores,
+//   S[l, v] += sum_pairs T[l, pair] * f[c+a][p][v] f[c+b][q][v],
+// i.e. the same contraction as the single-VDF kernel with instances in place
+// of cells.  FP64: v_mfma_f64_16x16x4_f64.  FP32-compensated: the coefficients
+// and f are rounded to FP32, products and the per-block sums run on
+// v_mfma_f32_16x16x4_f32, and the block sums are folded into an FP32 (hi, lo)
+// pair with the two-sum of Knuth; the item writes hi + lo in FP64.  A second
+// kernel adds the items of each (cell, instance) in a fixed order in FP64 and
+// applies 2/w (and the spatial operator's scale).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
 #include "dgv_internal.h"
 
+namespace dgv {
+
+struct BatchGridDev {
+  const double* pairs64;   // npairs * (D+1) doubles
+  const float* pairs32;    // npairs * (D+1) floats (FP32-compensated path)
+  const BlockHdr* blocks;
+  const int32_t* blist;    // per-cell block lists, concatenated
+  const double* gwts;
+  int64_t foff;            // offset of this grid in the packed buffers
+  int32_t Bpad, Mv, Mw, D;
+};
+
+struct BatchItem {
+  int32_t grid, cell, b0, b1;  // blist range of the segment
+  int32_t vc, pad;             // chunk of 64 instances
+  int64_t prow;                // first partial row (64 rows per item)
+};
+
+}  // namespace dgv
+
+struct dgv_batch {
+  std::vector<dgv_ctx*> grids;
+  int precision = DGV_FP64;
+  int32_t ninst = 0;
+  std::vector<int32_t> inst_grid, inst_local;
+  std::vector<int32_t> nin, Bpad;
+  std::vector<int64_t> foff;       // element offset of each grid's packed block
+  int64_t total = 0;               // packed elements
+  double* d_f = nullptr;
+  double* d_q = nullptr;
+  float* d_f32 = nullptr;
+  double* d_tmp[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int32_t*> d_blist;
+  std::vector<std::vector<int64_t>> cell_ptr;  // per grid: ncells+1 offsets into blist
+  std::vector<float*> d_pairs32;
+  dgv::BatchGridDev* d_grids = nullptr;
+  int32_t tile = 0, waves_per_cu = 0;
+  std::vector<dgv::BatchItem> items;
+  dgv::BatchItem* d_items = nullptr;
+  int64_t nrows = 0;
+  double* d_partial = nullptr;
+  // reduction plan: per (grid, cell, chunk) the partial rows in item order
+  std::vector<int64_t> red_ptr;
+  int32_t* d_red_rows = nullptr;
+  int64_t* d_red_ptr = nullptr;
+  int32_t* d_red_key = nullptr;    // (grid, cell, chunk) per reduction task, packed
+  int64_t nred = 0;
+  int* d_counter = nullptr;
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t graph = nullptr;
+  int graph_key[2] = {0, 0};
+  double graph_dt = 0, graph_coef = 0;
+  std::string err;
+};
+
+namespace dgv {
+
+static int berr(dgv_batch* b, int code, const std::string& m) {
+  if (b) b->err = m;
+  set_global_error(m);
+  return code;
+}
+#define BHIP(b, call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return berr((b), DGV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef double bd4 __attribute__((ext_vector_type(4)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_pairs_to_f32(int64_t npairs, int D, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs * (D + 1)) return;
+  const int l = (int)(i % (D + 1));
+  if (l < D) b[i] = (float)a[i];
+  else b[i] = __int_as_float((int)__double_as_longlong(a[i]));  // pair code
+}
+
+__global__ void k_reset_counter(int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+// Knuth's two-sum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+
+template <int D, bool FP32>
+__global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
+                                                   const BatchGridDev* __restrict__ grids,
+                                                   const double* __restrict__ f64, const float* __restrict__ f32,
+                                                   double* __restrict__ partial, int* __restrict__ counter) {
+  constexpr int RT = (D + 15) / 16;
+  using T = typename std::conditional<FP32, float, double>::type;
+  using V4 = typename std::conditional<FP32, bf4, bd4>::type;
+  __shared__ T lfa[D * 64], lfb[D * 64];
+  __shared__ int s_item;
+  const int lane = threadIdx.x;
+  const int kk = lane >> 4, jj = lane & 15;
+  for (;;) {
+    if (lane == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int id = s_item;
+    __syncthreads();
+    if (id >= nitems) return;
+    const BatchItem it = items[id];
+    const BatchGridDev G = grids[it.grid];
+    const int Mv = G.Mv, Mw = G.Mw;
+    const int c = it.cell;
+    const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
+    const int64_t vb = (int64_t)it.vc * 64 + lane;   // this lane's instance (staging)
+    V4 acc[RT][4], hi[RT][4], lo[RT][4];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        acc[rt][ct] = V4{0, 0, 0, 0};
+        hi[rt][ct] = V4{0, 0, 0, 0};
+        lo[rt][ct] = V4{0, 0, 0, 0};
+      }
+    for (int bi = it.b0; bi < it.b1; ++bi) {
+      const BlockHdr* B = G.blocks + G.blist[bi];
+      const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
+      const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+      __syncthreads();
+      if constexpr (FP32) {
+        const float* F = f32 + G.foff;
+#pragma unroll
+        for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
+#pragma unroll
+        for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
+      } else {
+        const double* F = f64 + G.foff;
+#pragma unroll
+        for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
+#pragma unroll
+        for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
+      }
+      __syncthreads();
+      const int n = B->npairs;
+      const T* P;
+      if constexpr (FP32) P = G.pairs32 + B->pbeg * (D + 1);
+      else P = G.pairs64 + B->pbeg * (D + 1);
+      auto load = [&](int s, T (&a)[RT], int& code) {
+        const int pr = s + kk;
+        const bool ok = pr < n;
+        const T* R = P + (int64_t)pr * (D + 1);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) a[rt] = (ok && 16 * rt + jj < D) ? R[16 * rt + jj] : (T)0;
+        if constexpr (FP32) code = ok ? __float_as_int(R[D]) : -1;
+        else code = ok ? (int)__double_as_longlong(R[D]) : -1;
+      };
+      T a0[RT], a1[RT];
+      int c0, c1;
+      load(0, a0, c0);
+      load(4, a1, c1);
+      for (int s = 0; s < n; s += 4) {
+        T a2[RT];
+        int c2;
+        load(s + 8, a2, c2);
+        const int p = c0 >> 8, q = c0 & 255;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          const int v = 16 * ct + jj;
+          const T b = c0 >= 0 ? lfa[p * 64 + v] * lfb[q * 64 + v] : (T)0;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            if constexpr (FP32)
+              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt], b, acc[rt][ct], 0, 0, 0);
+            else
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rt], b, acc[rt][ct], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) { a0[rt] = a1[rt]; a1[rt] = a2[rt]; }
+        c0 = c1;
+        c1 = c2;
+      }
+      if constexpr (FP32) {  // fold the block sum into the compensated accumulator
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              float s, er;
+              two_sum(hi[rt][ct][e], acc[rt][ct][e], s, er);
+              hi[rt][ct][e] = s;
+              lo[rt][ct][e] += er;
+              acc[rt][ct][e] = 0.0f;
+            }
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int v = 16 * ct + jj;
+        double* out = partial + (it.prow + v) * D;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // output layouts: FP64 row 4e + lane/16, FP32 row 4 (lane/16) + e
+          const int l = FP32 ? 16 * rt + 4 * kk + e : 16 * rt + 4 * e + kk;
+          if (l < D) {
+            if constexpr (FP32) out[l] = (double)hi[rt][ct][e] + (double)lo[rt][ct][e];
+            else out[l] = acc[rt][ct][e];
+          }
+        }
+      }
+  }
+}
+
+// One wave per (grid, cell, chunk): lane v sums its instance's rows over the
+// cell's items in order, Q = (2 s) / w * scale, written to the packed layout.
+template <int D>
+__global__ __launch_bounds__(64) void k_batch_reduce(int64_t ntask, const int32_t* __restrict__ key,
+                                                     const int64_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ rows,
+                                                     const BatchGridDev* __restrict__ grids,
+                                                     const double* __restrict__ partial, double scale,
+                                                     double* __restrict__ q) {
+  const int64_t t = blockIdx.x;
+  if (t >= ntask) return;
+  const int lane = threadIdx.x;
+  const int g = key[3 * t], c = key[3 * t + 1], vc = key[3 * t + 2];
+  const BatchGridDev G = grids[g];
+  double s[D];
+#pragma unroll
+  for (int l = 0; l < D; ++l) s[l] = 0.0;
+  for (int64_t k = ptr[t]; k < ptr[t + 1]; ++k) {
+    const double* row = partial + ((int64_t)rows[k] + lane) * D;
+#pragma unroll
+    for (int l = 0; l < D; ++l) s[l] = s[l] + row[l];
+  }
+  const int64_t v = (int64_t)vc * 64 + lane;
+  double* Q = q + G.foff;
+#pragma unroll
+  for (int l = 0; l < D; ++l) {
+    const int64_t node = (int64_t)c * D + l;
+    double x = (2.0 * s[l]) / G.gwts[node];
+    if (scale != 1.0) x = x * scale;
+    Q[node * G.Bpad + v] = x;
+  }
+}
+
+// out = t0 + t1 + ... over the packed buffers (as k_lincomb)
+struct BLinTerm {
+  const double* x;
+  double m1, den;
+  int neg;
+};
+struct BLinComb {
+  BLinTerm t[5];
+  int n;
+};
+__global__ void k_blincomb(int64_t n, double* __restrict__ out, BLinComb c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < c.n; ++k) {
+    double v = c.t[k].x[i];
+    if (c.t[k].m1 != 1.0) v = v * c.t[k].m1;
+    if (c.t[k].den != 1.0) v = v / c.t[k].den;
+    if (k == 0) s = c.t[k].neg ? -v : v;
+    else s = c.t[k].neg ? s - v : s + v;
+  }
+  out[i] = s;
+}
+
+static void destroy_plan(dgv_batch* b) {
+  hipFree(b->d_items); hipFree(b->d_partial); hipFree(b->d_red_rows); hipFree(b->d_red_ptr); hipFree(b->d_red_key);
+  b->d_items = nullptr; b->d_partial = nullptr; b->d_red_rows = nullptr; b->d_red_ptr = nullptr; b->d_red_key = nullptr;
+  b->items.clear();
+  if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+}
+
+// Per cell and grid: blocks whose box contains the cell, cut into segments of
+// about `tile` pairs; items = segments x chunks of 64 instances.
+static int build_batch_plan(dgv_batch* b, int32_t tile) {
+  destroy_plan(b);
+  if (tile <= 0) tile = 1 << 20;
+  b->tile = tile;
+  std::vector<BatchItem> items;
+  struct Task { int32_t g, c, vc; std::vector<int32_t> rows; };
+  std::vector<int32_t> key;
+  std::vector<int64_t> rptr(1, 0);
+  std::vector<int32_t> rrows;
+  for (size_t gi = 0; gi < b->grids.size(); ++gi) {
+    if (b->nin[gi] == 0) continue;
+    const dgv_ctx* ctx = b->grids[gi];
+    const PairBlock& pb = ctx->pb;
+    const std::vector<int64_t>& cp = b->cell_ptr[gi];
+    std::vector<int32_t> blist(cp.back());
+    {
+      // the same per-cell block order as in the device list
+      std::vector<int64_t> pos(cp.begin(), cp.end() - 1);
+      for (const Group& G : pb.groups)
+        for (int r = 0; r < G.ncells; ++r) {
+          int rr = r;
+          const int cw = G.lo[2] + rr % G.ext[2]; rr /= G.ext[2];
+          const int cv = G.lo[1] + rr % G.ext[1]; rr /= G.ext[1];
+          const int cu = G.lo[0] + rr;
+          const int64_t cell = ((int64_t)cu * ctx->g.Mv + cv) * ctx->g.Mw + cw;
+          for (int k = G.bbeg; k < G.bend; ++k) blist[pos[cell]++] = pb.group_blocks[k];
+        }
+    }
+    const int nchunk = b->Bpad[gi] / 64;
+    for (int64_t c = 0; c < ctx->g.ncells; ++c) {
+      std::vector<std::pair<int32_t, int32_t>> segs;
+      int64_t s0 = cp[c], acc = 0;
+      for (int64_t k = cp[c]; k < cp[c + 1]; ++k) {
+        acc += pb.blocks[blist[k]].npairs;
+        if (acc >= tile || k + 1 == cp[c + 1]) {
+          segs.push_back({(int32_t)s0, (int32_t)(k + 1)});
+          s0 = k + 1;
+          acc = 0;
+        }
+      }
+      for (int vc = 0; vc < nchunk; ++vc) {
+        key.push_back((int32_t)gi); key.push_back((int32_t)c); key.push_back(vc);
+        for (auto& sg : segs) {
+          BatchItem it;
+          it.grid = (int32_t)gi; it.cell = (int32_t)c; it.b0 = sg.first; it.b1 = sg.second;
+          it.vc = vc; it.pad = 0;
+          it.prow = (int64_t)items.size() * 64;
+          if (it.prow > INT32_MAX - 64) return berr(b, DGV_ERR_UNSUPPORTED, "too many batch work items");
+          rrows.push_back((int32_t)it.prow);
+          items.push_back(it);
+        }
+        rptr.push_back((int64_t)rrows.size());
+      }
+    }
+  }
+  const int D = b->grids[0]->g.D;
+  // keep the partial sums within 4 GiB: coarser segments otherwise
+  if ((int64_t)items.size() * 64 * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30)) return build_batch_plan(b, tile * 2);
+  b->items.swap(items);
+  b->nrows = (int64_t)b->items.size() * 64;
+  b->nred = (int64_t)rptr.size() - 1;
+  BHIP(b, hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(b->items.size(), 1)));
+  BHIP(b, hipMemcpy(b->d_items, b->items.data(), sizeof(BatchItem) * b->items.size(), hipMemcpyHostToDevice));
+  if (hipMalloc(&b->d_partial, sizeof(double) * std::max<int64_t>(b->nrows * D, 1)) != hipSuccess)
+    return berr(b, DGV_ERR_NOMEM, "hipMalloc(batch partial sums) failed");
+  BHIP(b, hipMalloc(&b->d_red_rows, sizeof(int32_t) * std::max<size_t>(rrows.size(), 1)));
+  BHIP(b, hipMemcpy(b->d_red_rows, rrows.data(), sizeof(int32_t) * rrows.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMalloc(&b->d_red_ptr, sizeof(int64_t) * rptr.size()));
+  BHIP(b, hipMemcpy(b->d_red_ptr, rptr.data(), sizeof(int64_t) * rptr.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMalloc(&b->d_red_key, sizeof(int32_t) * std::max<size_t>(key.size(), 1)));
+  BHIP(b, hipMemcpy(b->d_red_key, key.data(), sizeof(int32_t) * key.size(), hipMemcpyHostToDevice));
+  return DGV_OK;
+}
+
+static int batch_eval_dev(dgv_batch* b, const double* d_f, double* d_q, double scale) {
+  const int D = b->grids[0]->g.D;
+  const int nitems = (int)b->items.size();
+  hipStream_t st = b->stream;
+  if (b->precision == DGV_FP32_COMP)
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
+  const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
+  const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc));
+  if (nitems > 0) {
+    if (D == 27) {
+      if (b->precision == DGV_FP32_COMP)
+        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, b->d_partial, b->d_counter);
+      else
+        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, b->d_partial, b->d_counter);
+    } else {
+      if (b->precision == DGV_FP32_COMP)
+        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, b->d_partial, b->d_counter);
+      else
+        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, b->d_partial, b->d_counter);
+    }
+  }
+  if (b->nred > 0) {
+    if (D == 27)
+      hipLaunchKernelGGL(k_batch_reduce<27>, dim3((unsigned)b->nred), dim3(64), 0, st, b->nred, b->d_red_key,
+                         b->d_red_ptr, b->d_red_rows, b->d_grids, b->d_partial, scale, d_q);
+    else
+      hipLaunchKernelGGL(k_batch_reduce<8>, dim3((unsigned)b->nred), dim3(64), 0, st, b->nred, b->d_red_key,
+                         b->d_red_ptr, b->d_red_rows, b->d_grids, b->d_partial, scale, d_q);
+  }
+  BHIP(b, hipGetLastError());
+  return DGV_OK;
+}
+
+static int blincomb(dgv_batch* b, double* out, int n, const double* const* xs, const double* m1, const double* den,
+                    const int* neg) {
+  BLinComb c;
+  std::memset(&c, 0, sizeof(c));
+  c.n = n;
+  for (int k = 0; k < n; ++k) { c.t[k].x = xs[k]; c.t[k].m1 = m1[k]; c.t[k].den = den[k]; c.t[k].neg = neg[k]; }
+  hipLaunchKernelGGL(k_blincomb, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, b->stream, b->total, out, c);
+  BHIP(b, hipGetLastError());
+  return DGV_OK;
+}
+
+// One Runge-Kutta step (rkmts 1..3) of every instance, the reference's
+// combinations (time_integr_mod.f90:37-83) on the packed buffers.
+static int batch_rk_step(dgv_batch* b, double dt, double coef, int rkmts) {
+  double *f = b->d_f, *k1 = b->d_tmp[0], *k2 = b->d_tmp[1], *k3 = b->d_tmp[2], *tmp = b->d_tmp[3];
+  const double ct = coef * dt;
+  int rc;
+#define BSOP(in, out) do { rc = batch_eval_dev(b, in, out, ct); if (rc) return rc; } while (0)
+  switch (rkmts) {
+    case 1: {
+      BSOP(f, k1);
+      const double* x[] = {f, k1}; double m[] = {1, 1}, d[] = {1, 1}; int ng[] = {0, 0};
+      return blincomb(b, f, 2, x, m, d, ng);
+    }
+    case 2: {
+      BSOP(f, k1);
+      { const double* x[] = {f, k1}; double m[] = {1, 2}, d[] = {1, 3.0}; int ng[] = {0, 0};
+        if ((rc = blincomb(b, tmp, 2, x, m, d, ng))) return rc; }
+      BSOP(tmp, k2);
+      const double* x[] = {f, k1, k2}; double m[] = {1, 1, 3.0}, d[] = {1, 4.0, 4}; int ng[] = {0, 0, 0};
+      return blincomb(b, f, 3, x, m, d, ng);
+    }
+    case 3: {
+      BSOP(f, k1);
+      { const double* x[] = {f, k1}; double m[] = {1, 1}, d[] = {1, 2.0}; int ng[] = {0, 0};
+        if ((rc = blincomb(b, tmp, 2, x, m, d, ng))) return rc; }
+      BSOP(tmp, k2);
+      { const double* x[] = {f, k2}; double m[] = {1, 3.0}, d[] = {1, 4}; int ng[] = {0, 0};
+        if ((rc = blincomb(b, tmp, 2, x, m, d, ng))) return rc; }
+      BSOP(tmp, k3);
+      const double* x[] = {f, k1, k2, k3}; double m[] = {1, 2, 3.0, 4}, d[] = {1, 9.0, 9.0, 9.0};
+      int ng[] = {0, 0, 0, 0};
+      return blincomb(b, f, 4, x, m, d, ng);
+    }
+  }
+#undef BSOP
+  return berr(b, DGV_ERR_ARG, "batched stepping supports rkmts 1..3");
+}
+
+}  // namespace dgv
+
+using namespace dgv;
+
 extern "C" {
-int dgv_batch_create(dgv_ctx* const*, int32_t, const int32_t*, int32_t, int32_t, dgv_batch** out) {
-  if (out) *out = nullptr;
-  dgv::set_global_error("batched evaluation not built yet");
-  return DGV_ERR_UNSUPPORTED;
+
+int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_of_instance, int32_t ninstances,
+                     int32_t precision, dgv_batch** out) {
+  if (!out || !grids || ngrids < 1 || !grid_of_instance || ninstances < 1) {
+    set_global_error("dgv_batch_create: bad argument");
+    return DGV_ERR_ARG;
+  }
+  *out = nullptr;
+  if (precision != DGV_FP64 && precision != DGV_FP32_COMP) {
+    set_global_error("precision must be DGV_FP64 or DGV_FP32_COMP");
+    return DGV_ERR_ARG;
+  }
+  const int D = grids[0] ? grids[0]->g.D : 0;
+  for (int g = 0; g < ngrids; ++g) {
+    if (!grids[g] || !grids[g]->has_a) { set_global_error("every grid needs an A-array"); return DGV_ERR_STATE; }
+    if (grids[g]->g.D != D || (D != 27 && D != 8)) {
+      set_global_error("batched evaluation needs 8 or 27 nodes per cell on every grid");
+      return DGV_ERR_UNSUPPORTED;
+    }
+    if (grids[g]->device != grids[0]->device) { set_global_error("all grids must be on one device"); return DGV_ERR_ARG; }
+  }
+  dgv_batch* b = new dgv_batch();
+  b->grids.assign(grids, grids + ngrids);
+  b->precision = precision;
+  b->ninst = ninstances;
+  b->nin.assign(ngrids, 0);
+  b->inst_grid.resize(ninstances);
+  b->inst_local.resize(ninstances);
+  for (int i = 0; i < ninstances; ++i) {
+    const int g = grid_of_instance[i];
+    if (g < 0 || g >= ngrids) { delete b; set_global_error("grid_of_instance out of range"); return DGV_ERR_ARG; }
+    b->inst_grid[i] = g;
+    b->inst_local[i] = b->nin[g]++;
+  }
+  b->Bpad.resize(ngrids);
+  b->foff.resize(ngrids);
+  int64_t off = 0;
+  for (int g = 0; g < ngrids; ++g) {
+    b->Bpad[g] = ((b->nin[g] + 63) / 64) * 64;
+    b->foff[g] = off;
+    off += grids[g]->g.N * (int64_t)b->Bpad[g];
+  }
+  b->total = off;
+  auto fail = [&](int code, const std::string& m) { dgv_batch_destroy(b); set_global_error(m); return code; };
+  if (hipSetDevice(grids[0]->device) != hipSuccess) return fail(DGV_ERR_HIP, "hipSetDevice failed");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, grids[0]->device) == hipSuccess) b->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(DGV_ERR_HIP, "stream");
+  for (int i = 0; i < 3; ++i)
+    if (hipEventCreate(&b->ev[i]) != hipSuccess) return fail(DGV_ERR_HIP, "event");
+  const size_t nb = sizeof(double) * std::max<int64_t>(b->total, 1);
+  if (hipMalloc(&b->d_f, nb) || hipMalloc(&b->d_q, nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
+  for (int i = 0; i < 4; ++i)
+    if (hipMalloc(&b->d_tmp[i], nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
+  if (hipMemset(b->d_f, 0, nb) || hipMemset(b->d_q, 0, nb)) return fail(DGV_ERR_HIP, "hipMemset");
+  if (precision == DGV_FP32_COMP && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
+    return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
+  if (hipMalloc(&b->d_counter, sizeof(int) * 4)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
+  // per grid: per-cell block lists, FP32 coefficients
+  std::vector<BatchGridDev> gd(ngrids);
+  b->d_blist.assign(ngrids, nullptr);
+  b->d_pairs32.assign(ngrids, nullptr);
+  b->cell_ptr.assign(ngrids, {});
+  for (int g = 0; g < ngrids; ++g) {
+    const dgv_ctx* ctx = grids[g];
+    const PairBlock& pb = ctx->pb;
+    std::vector<int64_t> cnt(ctx->g.ncells + 1, 0);
+    for (const Group& G : pb.groups)
+      for (int r = 0; r < G.ncells; ++r) {
+        int rr = r;
+        const int cw = G.lo[2] + rr % G.ext[2]; rr /= G.ext[2];
+        const int cv = G.lo[1] + rr % G.ext[1]; rr /= G.ext[1];
+        const int cu = G.lo[0] + rr;
+        cnt[((int64_t)cu * ctx->g.Mv + cv) * ctx->g.Mw + cw + 1] += G.bend - G.bbeg;
+      }
+    for (int64_t c = 0; c < ctx->g.ncells; ++c) cnt[c + 1] += cnt[c];
+    std::vector<int32_t> blist(cnt.back());
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (const Group& G : pb.groups)
+      for (int r = 0; r < G.ncells; ++r) {
+        int rr = r;
+        const int cw = G.lo[2] + rr % G.ext[2]; rr /= G.ext[2];
+        const int cv = G.lo[1] + rr % G.ext[1]; rr /= G.ext[1];
+        const int cu = G.lo[0] + rr;
+        const int64_t cell = ((int64_t)cu * ctx->g.Mv + cv) * ctx->g.Mw + cw;
+        for (int k = G.bbeg; k < G.bend; ++k) blist[pos[cell]++] = pb.group_blocks[k];
+      }
+    b->cell_ptr[g] = cnt;
+    if (hipMalloc(&b->d_blist[g], sizeof(int32_t) * std::max<size_t>(blist.size(), 1)) ||
+        hipMemcpy(b->d_blist[g], blist.data(), sizeof(int32_t) * blist.size(), hipMemcpyHostToDevice))
+      return fail(DGV_ERR_NOMEM, "block lists");
+    if (precision == DGV_FP32_COMP) {
+      const int64_t n = pb.npairs * (D + 1);
+      if (hipMalloc(&b->d_pairs32[g], sizeof(float) * std::max<int64_t>(n, 1)))
+        return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 coefficients)");
+      hipLaunchKernelGGL(k_pairs_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, pb.npairs, D,
+                         pb.d_pairs, b->d_pairs32[g]);
+    }
+    gd[g].pairs64 = pb.d_pairs;
+    gd[g].pairs32 = b->d_pairs32[g];
+    gd[g].blocks = pb.d_blocks;
+    gd[g].blist = b->d_blist[g];
+    gd[g].gwts = ctx->d_gwts;
+    gd[g].foff = b->foff[g];
+    gd[g].Bpad = b->Bpad[g];
+    gd[g].Mv = ctx->g.Mv;
+    gd[g].Mw = ctx->g.Mw;
+    gd[g].D = D;
+  }
+  if (hipMalloc(&b->d_grids, sizeof(BatchGridDev) * ngrids) ||
+      hipMemcpy(b->d_grids, gd.data(), sizeof(BatchGridDev) * ngrids, hipMemcpyHostToDevice))
+    return fail(DGV_ERR_HIP, "grid table");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(DGV_ERR_HIP, "coefficient conversion");
+  int rc = build_batch_plan(b, 0);
+  if (rc) { std::string m = b->err; dgv_batch_destroy(b); set_global_error(m); return rc; }
+  *out = b;
+  return DGV_OK;
 }
-void dgv_batch_destroy(dgv_batch*) {}
-int dgv_batch_layout(const dgv_batch*, int32_t, int64_t*, int64_t*, int64_t*) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_set_f(dgv_batch*, int32_t, const double*, int64_t) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_get_q(dgv_batch*, int32_t, double*, int64_t) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_eval_q(dgv_batch*) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_rk_run(dgv_batch*, double, double, int32_t, int64_t, int) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_autotune(dgv_batch*, int32_t*, double*) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_device_buffers(dgv_batch*, double**, double**) { return DGV_ERR_UNSUPPORTED; }
-int dgv_batch_time_eval(dgv_batch*, int32_t, double*, double*) { return DGV_ERR_UNSUPPORTED; }
+
+void dgv_batch_destroy(dgv_batch* b) {
+  if (!b) return;
+  if (!b->grids.empty()) hipSetDevice(b->grids[0]->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  destroy_plan(b);
+  hipFree(b->d_f); hipFree(b->d_q); hipFree(b->d_f32); hipFree(b->d_counter); hipFree(b->d_grids);
+  for (int i = 0; i < 4; ++i) hipFree(b->d_tmp[i]);
+  for (auto p : b->d_blist) hipFree(p);
+  for (auto p : b->d_pairs32) hipFree(p);
+  for (int i = 0; i < 3; ++i)
+    if (b->ev[i]) hipEventDestroy(b->ev[i]);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
 }
+
+int dgv_batch_layout(const dgv_batch* b, int32_t instance, int64_t* offset, int64_t* stride, int64_t* total) {
+  if (!b || instance < 0 || instance >= b->ninst) { set_global_error("bad batch or instance"); return DGV_ERR_ARG; }
+  const int g = b->inst_grid[instance];
+  if (offset) *offset = b->foff[g] + b->inst_local[instance];
+  if (stride) *stride = b->Bpad[g];
+  if (total) *total = b->total;
+  return DGV_OK;
+}
+
+static int batch_copy(dgv_batch* b, int32_t instance, double* host, int64_t n, bool to_device, double* dev) {
+  if (!b || instance < 0 || instance >= b->ninst || !host) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  const int g = b->inst_grid[instance];
+  const int64_t N = b->grids[g]->g.N;
+  if (n != N) return berr(b, DGV_ERR_ARG, "array length differs from the instance's node count");
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  // strided copy through a 2-D memcpy: N rows of one double, pitch Bpad
+  double* base = dev + b->foff[g] + b->inst_local[instance];
+  const size_t pitch = sizeof(double) * b->Bpad[g];
+  if (to_device)
+    BHIP(b, hipMemcpy2DAsync(base, pitch, host, sizeof(double), sizeof(double), N, hipMemcpyHostToDevice, b->stream));
+  else
+    BHIP(b, hipMemcpy2DAsync(host, sizeof(double), base, pitch, sizeof(double), N, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_set_f(dgv_batch* b, int32_t instance, const double* f, int64_t n) {
+  return batch_copy(b, instance, const_cast<double*>(f), n, true, b ? b->d_f : nullptr);
+}
+
+int dgv_batch_get_q(dgv_batch* b, int32_t instance, double* q, int64_t n) {
+  return batch_copy(b, instance, q, n, false, b ? b->d_q : nullptr);
+}
+
+int dgv_batch_eval_q(dgv_batch* b) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  int rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+  if (rc) return rc;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_t nsteps, int use_graph) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  if (rkmts < 1 || rkmts > 3) return berr(b, DGV_ERR_ARG, "batched stepping supports rkmts 1..3");
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  if (!use_graph) {
+    for (int64_t s = 0; s < nsteps; ++s) {
+      int rc = batch_rk_step(b, dt, coef, rkmts);
+      if (rc) return rc;
+    }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return DGV_OK;
+  }
+  const bool same = b->graph && b->graph_key[0] == rkmts && b->graph_key[1] == b->tile && b->graph_dt == dt &&
+                    b->graph_coef == coef;
+  if (!same) {
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    hipGraph_t graph;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_rk_step(b, dt, coef, rkmts);
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return berr(b, DGV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return berr(b, DGV_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->graph_key[0] = rkmts;
+    b->graph_key[1] = b->tile;
+    b->graph_dt = dt;
+    b->graph_coef = coef;
+  }
+  for (int64_t s = 0; s < nsteps; ++s) BHIP(b, hipGraphLaunch(b->graph, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_time_eval(dgv_batch* b, int32_t niter, double* ms_per_call, double* ms_main_kernel) {
+  if (!b || niter < 1) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  BHIP(b, hipEventRecord(b->ev[0], b->stream));
+  for (int i = 0; i < niter; ++i) {
+    int rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+    if (rc) return rc;
+  }
+  BHIP(b, hipEventRecord(b->ev[1], b->stream));
+  BHIP(b, hipEventSynchronize(b->ev[1]));
+  float a = 0;
+  BHIP(b, hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+  if (ms_per_call) *ms_per_call = a / niter;
+  if (ms_main_kernel) *ms_main_kernel = a / niter;
+  return DGV_OK;
+}
+
+int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  const int32_t tiles[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
+  const int32_t wpcs[] = {4, 8, 16};
+  int32_t bt = 0, bw = 0;
+  double bms = 1e300;
+  for (int32_t t : tiles) {
+    int rc = build_batch_plan(b, t);
+    if (rc) return rc;
+    for (int32_t w : wpcs) {
+      b->waves_per_cu = w;
+      double ms;
+      rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+      if (rc) return rc;
+      rc = dgv_batch_time_eval(b, 2, &ms, nullptr);
+      if (rc) return rc;
+      if (ms < bms) { bms = ms; bt = t; bw = w; }
+    }
+  }
+  b->waves_per_cu = bw;
+  int rc = build_batch_plan(b, bt);
+  if (rc) return rc;
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return DGV_OK;
+}
+
+int dgv_batch_device_buffers(dgv_batch* b, double** d_f, double** d_q) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  if (d_f) *d_f = b->d_f;
+  if (d_q) *d_q = b->d_q;
+  return DGV_OK;
+}
+
+}  // extern "C"

@@ -295,7 +295,7 @@ int dgv_aarray_chunk_name(const dgv_grid_desc* grid, const char* dir, const char
 
 int dgv_ctx_set_kernel(dgv_ctx* ctx, int kernel) {
   CHECK_CTX(ctx);
-  if (kernel < DGV_KERNEL_AUTO || kernel > DGV_KERNEL_RECORD) return set_error(ctx, DGV_ERR_ARG, "unknown kernel");
+  if (kernel < DGV_KERNEL_AUTO || kernel > DGV_KERNEL_MFMA) return set_error(ctx, DGV_ERR_ARG, "unknown kernel");
   ctx->kernel = kernel;
   if (ctx->rk_graph) { hipGraphExecDestroy(ctx->rk_graph); ctx->rk_graph = nullptr; }
   return DGV_OK;

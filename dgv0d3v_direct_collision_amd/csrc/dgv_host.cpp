@@ -357,7 +357,7 @@ int build_pairblock_from_records(dgv_ctx* ctx, const int32_t* capphi, int64_t nc
       pairs[base + ent[j].l] += ent[j].A;
       ++j;
     }
-    int64_t qbits = (int64_t)(ent[i].key & 63);
+    int64_t qbits = (int64_t)((((ent[i].key >> 6) & 63) << 8) | (ent[i].key & 63));  // pair code p<<8|q
     std::memcpy(&pairs[base + D], &qbits, sizeof(double));
     i = j;
   }
@@ -502,6 +502,7 @@ static int export_records_uncached(const dgv_ctx* ctx, std::vector<std::vector<s
         const double* R = &pairs[(size_t)(B.pbeg + j) * (D + 1)];
         int64_t q;
         std::memcpy(&q, &R[D], sizeof(q));
+        q &= 255;
         for (int l = 0; l < D; ++l)
           if (R[l] != 0.0) {
             (*idx)[l].push_back({xc * D + p, yc * D + q});
@@ -598,6 +599,7 @@ static int export_one_basis(const dgv_ctx* ctx, int l, std::vector<std::array<in
         if (R[l] == 0.0) continue;
         int64_t q;
         std::memcpy(&q, &R[D], sizeof(q));
+        q &= 255;
         idx.push_back({xc * D + p, yc * D + q});
         val.push_back(R[l]);
       }

@@ -9,8 +9,8 @@
 // pairs exist (dgvtools_mod.f90:493-530) does not depend on l.  On the device the
 // records are therefore regrouped by pair:
 //
-//   pair   = (q, T[0..D-1])        T[l] = A of basis l (0 when l has no record),
-//                                   q = local index of xi1 in its cell;
+//   pair   = (T[0..D-1], code)     T[l] = A of basis l (0 when l has no record),
+//                                   code = p << 8 | q, p / q = local index of xi / xi1;
 //   block  = all pairs whose xi lies in cell (cc + a) and xi1 in cell (cc + b),
 //            sorted by p = local index of xi; seg[p] = first pair with index >= p;
 //   group  = all blocks sharing the same box of output cells c for which both
@@ -73,7 +73,7 @@ struct PairBlock {
   std::vector<Group> groups;
   std::vector<int32_t> group_blocks;  // block indices grouped by box
   // device copies
-  double* d_pairs = nullptr;          // npairs * (D+1) doubles: T[0..D-1], q bits
+  double* d_pairs = nullptr;          // npairs * (D+1) doubles: T[0..D-1], code bits
   BlockHdr* d_blocks = nullptr;
   int32_t* d_group_blocks = nullptr;
   Group* d_groups = nullptr;
@@ -161,8 +161,8 @@ void free_plan(dgv_ctx* ctx);
 int ensure_record_layout(dgv_ctx* ctx);
 
 // kernels (dgv_kernels.hip)
-int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
-int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q);
+int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale, bool mfma);
+int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
 int launch_axpy(dgv_ctx* ctx, int64_t n, double* out, const double* x0, int nterms,
                 const double* const* xs, const double* coefs);

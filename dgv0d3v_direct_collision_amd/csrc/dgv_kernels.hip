@@ -78,11 +78,11 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
       const double fap = fa[p];
       // q of the next pair is read together with the current coefficients,
       // so each pair costs one scalar-load round trip
-      int qn = j0 < j1 ? (int)__double_as_longlong(P[(int64_t)j0 * (D + 1) + D]) : 0;
+      int qn = j0 < j1 ? (int)(__double_as_longlong(P[(int64_t)j0 * (D + 1) + D]) & 255) : 0;
       for (int j = j0; j < j1; ++j) {
         const double* R = P + (int64_t)j * (D + 1);
         const double fbq = fb[qn * 64 + lane];
-        if (j + 1 < j1) qn = (int)__double_as_longlong(R[(D + 1) + D]);
+        if (j + 1 < j1) qn = (int)(__double_as_longlong(R[(D + 1) + D]) & 255);
         const double gq = fap * fbq;
 #pragma unroll
         for (int l = 0; l < D; ++l) acc[l] = __fma_rn(R[l], gq, acc[l]);
@@ -96,6 +96,108 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
     for (int l = 0; l < D; ++l) out[l] = acc[l];
   }
   if (sink == 0x5eed5eed && it.prow < 0) partial[0] = (double)sink;  // never true; keeps the prefetch
+}
+
+// Pair-block evaluator on the matrix cores.  For one block and the 64 cells
+// of a chunk the sum is a small matrix product,
+//   Q[l, cell] += sum_pairs T[l, pair] * g[pair, cell],
+//   g[pair, cell] = f[cell + a][p] * f[cell + b][q],
+// done with v_mfma_f64_16x16x4_f64 (A = 16 basis functions x 4 pairs,
+// B = 4 pairs x 16 cells, C = 16 x 16): RT row tiles cover the D basis
+// functions (rows >= D are zero), four column tiles the 64 cells.  The
+// coefficients reach the A operand through ordinary vector loads (lane i
+// holds T[pair s + i/16][16 rt + i%16]), issued two steps ahead; g is formed
+// on the VALU from the block's f values staged in LDS.  Operand layouts of
+// the instruction: A[i][k] and B[k][j] in lane i|j + 16 k; D[i][j] in lane
+// j + 16 (i%4), element i/4 (the FP64 output layout).
+typedef double dgv_d4 __attribute__((ext_vector_type(4)));
+
+template <int D>
+__global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ items,
+                                                const Group* __restrict__ groups,
+                                                const int32_t* __restrict__ gblocks,
+                                                const BlockHdr* __restrict__ blocks,
+                                                const double* __restrict__ pairs,
+                                                const double* __restrict__ f,
+                                                double* __restrict__ partial, int Mv, int Mw) {
+  constexpr int RT = (D + 15) / 16;
+  __shared__ double lfa[D * 64], lfb[D * 64];
+  const WorkItem it = items[blockIdx.x];
+  if (it.chunk < 0) return;
+  const Group g = groups[it.group];
+  const int lane = threadIdx.x;
+  const int cl = it.chunk * 64 + lane;
+  int r = cl < g.ncells ? cl : 0;   // idle lanes stage the box's first cell (in range)
+  const int cw = g.lo[2] + r % g.ext[2];
+  r /= g.ext[2];
+  const int cv = g.lo[1] + r % g.ext[1];
+  r /= g.ext[1];
+  const int cu = g.lo[0] + r;
+  const int kk = lane >> 4, jj = lane & 15;
+
+  dgv_d4 acc[RT][4];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = dgv_d4{0.0, 0.0, 0.0, 0.0};
+
+  for (int bi = it.bbeg; bi < it.bend; ++bi) {
+    const BlockHdr* B = blocks + gblocks[bi];
+    const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
+    const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+    __syncthreads();  // previous block's LDS reads are done
+#pragma unroll
+    for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
+#pragma unroll
+    for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
+    __syncthreads();
+    const double* P = pairs + B->pbeg * (D + 1);
+    const int n = B->npairs;
+    // operands of step s: this lane's pair is s + kk
+    auto load = [&](int s, double (&a)[RT], int& code) {
+      const int pr = s + kk;
+      const bool ok = pr < n;
+      const double* R = P + (int64_t)pr * (D + 1);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) a[rt] = (ok && 16 * rt + jj < D) ? R[16 * rt + jj] : 0.0;
+      code = ok ? (int)__double_as_longlong(R[D]) : -1;
+    };
+    double a0[RT], a1[RT];
+    int c0, c1;
+    load(0, a0, c0);
+    load(4, a1, c1);
+    for (int s = 0; s < n; s += 4) {
+      double a2[RT];
+      int c2;
+      load(s + 8, a2, c2);
+      const int p = c0 >> 8, q = c0 & 255;
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int cell = 16 * ct + jj;
+        const double b = c0 >= 0 ? lfa[p * 64 + cell] * lfb[q * 64 + cell] : 0.0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rt], b, acc[rt][ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) { a0[rt] = a1[rt]; a1[rt] = a2[rt]; }
+      c0 = c1;
+      c1 = c2;
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const int cell = 16 * ct + jj;
+      if (it.chunk * 64 + cell >= g.ncells) continue;
+      double* out = partial + (it.prow + cell) * D;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int l = 16 * rt + 4 * e + kk;
+        if (l < D) out[l] = acc[rt][ct][e];
+      }
+    }
 }
 
 // Final FP64 reduction, one wave per output cell: lane j sums the cell's
@@ -181,63 +283,29 @@ __global__ __launch_bounds__(256) void k_rec_eval(int D, int Mu, int Mv, int Mw,
   }
 }
 
-int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
+static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma) {
   const Grid& g = ctx->g;
-  const PairBlock& pb = ctx->pb;
-  const Plan& P = ctx->plan;
-  const int nitems = (int)P.items.size();
-  if (nitems > 0) {
-    switch (g.D) {
-#define DGV_PB_CASE(DD)                                                                          \
-  case DD:                                                                                       \
-    hipLaunchKernelGGL(k_pb_eval<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
-                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
-                       g.Mv, g.Mw);                                                             \
-    break;
-      DGV_PB_CASE(1)
-      DGV_PB_CASE(8)
-      DGV_PB_CASE(27)
-#undef DGV_PB_CASE
-      default:
-        return set_error(ctx, DGV_ERR_UNSUPPORTED, "pair-block kernel supports 1, 8 or 27 nodes per cell");
-    }
-  }
-  switch (g.D) {
-#define DGV_RED_CASE(DD)                                                                               \
-  case DD:                                                                                             \
-    hipLaunchKernelGGL(k_pb_reduce<DD>, dim3((unsigned)g.ncells), dim3(64), 0, ctx->stream, g.ncells,  \
-                       P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);             \
-    break;
-    DGV_RED_CASE(1)
-    DGV_RED_CASE(8)
-    DGV_RED_CASE(27)
-#undef DGV_RED_CASE
-  }
-  DGV_HIP(ctx, hipGetLastError());
-  return DGV_OK;
-}
-
-int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
-  int rc = ensure_record_layout(ctx);
-  if (rc) return rc;
-  const Grid& g = ctx->g;
-  hipLaunchKernelGGL(k_rec_eval, dim3((unsigned)g.N), dim3(256), 0, ctx->stream, g.D, g.Mu, g.Mv, g.Mw, g.ccu,
-                     g.ccv, g.ccw, ctx->d_rec_off, ctx->d_rec_A, ctx->d_rec_xi, ctx->d_rec_xi1, d_f, ctx->d_gwts,
-                     scale, d_q);
-  DGV_HIP(ctx, hipGetLastError());
-  return DGV_OK;
-}
-
-// The dominant kernel alone (no final reduction), for timing.
-int launch_eval_main_only(dgv_ctx* ctx, const double* d_f) {
-  int k = ctx->kernel;
-  const Grid& g = ctx->g;
-  if (k == DGV_KERNEL_AUTO) k = (g.D == 1 || g.D == 8 || g.D == 27) ? DGV_KERNEL_PAIRBLOCK : DGV_KERNEL_RECORD;
-  if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, ctx->d_q, 1.0);
   const PairBlock& pb = ctx->pb;
   const Plan& P = ctx->plan;
   const int nitems = (int)P.items.size();
   if (nitems == 0) return DGV_OK;
+  if (mfma) {
+    switch (g.D) {
+#define DGV_MF_CASE(DD)                                                                          \
+  case DD:                                                                                       \
+    hipLaunchKernelGGL(k_pb_mfma<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
+                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
+                       g.Mv, g.Mw);                                                             \
+    break;
+      DGV_MF_CASE(8)
+      DGV_MF_CASE(27)
+#undef DGV_MF_CASE
+      default:
+        return set_error(ctx, DGV_ERR_UNSUPPORTED, "matrix-core kernel supports 8 or 27 nodes per cell");
+    }
+    DGV_HIP(ctx, hipGetLastError());
+    return DGV_OK;
+  }
   switch (g.D) {
 #define DGV_PB_CASE(DD)                                                                          \
   case DD:                                                                                       \
@@ -256,14 +324,66 @@ int launch_eval_main_only(dgv_ctx* ctx, const double* d_f) {
   return DGV_OK;
 }
 
+static int launch_reduce(dgv_ctx* ctx, double* d_q, double scale) {
+  const Grid& g = ctx->g;
+  const Plan& P = ctx->plan;
+  switch (g.D) {
+#define DGV_RED_CASE(DD)                                                                               \
+  case DD:                                                                                             \
+    hipLaunchKernelGGL(k_pb_reduce<DD>, dim3((unsigned)g.ncells), dim3(64), 0, ctx->stream, g.ncells,  \
+                       P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);             \
+    break;
+    DGV_RED_CASE(1)
+    DGV_RED_CASE(8)
+    DGV_RED_CASE(27)
+#undef DGV_RED_CASE
+  }
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
+int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale, bool mfma) {
+  int rc = launch_main_pairblock(ctx, d_f, mfma);
+  if (rc) return rc;
+  return launch_reduce(ctx, d_q, scale);
+}
+
+int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
+  int rc = ensure_record_layout(ctx);
+  if (rc) return rc;
+  const Grid& g = ctx->g;
+  hipLaunchKernelGGL(k_rec_eval, dim3((unsigned)g.N), dim3(256), 0, ctx->stream, g.D, g.Mu, g.Mv, g.Mw, g.ccu,
+                     g.ccv, g.ccw, ctx->d_rec_off, ctx->d_rec_A, ctx->d_rec_xi, ctx->d_rec_xi1, d_f, ctx->d_gwts,
+                     scale, d_q);
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
 static bool pairblock_supported(int D) { return D == 1 || D == 8 || D == 27; }
+static bool mfma_supported(int D) { return D == 8 || D == 27; }
+
+static int resolve_kernel(const dgv_ctx* ctx) {
+  int k = ctx->kernel;
+  const int D = ctx->g.D;
+  if (k == DGV_KERNEL_AUTO)
+    k = mfma_supported(D) ? DGV_KERNEL_MFMA : (pairblock_supported(D) ? DGV_KERNEL_PAIRBLOCK : DGV_KERNEL_RECORD);
+  if (k == DGV_KERNEL_MFMA && !mfma_supported(D)) k = DGV_KERNEL_PAIRBLOCK;
+  if (k == DGV_KERNEL_PAIRBLOCK && !pairblock_supported(D)) k = DGV_KERNEL_RECORD;
+  return k;
+}
+
+// The dominant kernel alone (no final reduction), for timing.
+int launch_eval_main_only(dgv_ctx* ctx, const double* d_f) {
+  const int k = resolve_kernel(ctx);
+  if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, ctx->d_q, 1.0);
+  return launch_main_pairblock(ctx, d_f, k == DGV_KERNEL_MFMA);
+}
 
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set (dgv_ctx_set_aarray / generate)");
-  int k = ctx->kernel;
-  if (k == DGV_KERNEL_AUTO) k = pairblock_supported(ctx->g.D) ? DGV_KERNEL_PAIRBLOCK : DGV_KERNEL_RECORD;
-  if (k == DGV_KERNEL_PAIRBLOCK) return launch_eval_pairblock(ctx, d_f, d_q, scale);
-  return launch_eval_record(ctx, d_f, d_q, scale);
+  const int k = resolve_kernel(ctx);
+  if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, d_q, scale);
+  return launch_eval_pairblock(ctx, d_f, d_q, scale, k == DGV_KERNEL_MFMA);
 }
 
 // Nonzero coefficients per block (records of the reference layout).

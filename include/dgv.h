@@ -37,7 +37,7 @@ enum {
 };
 
 enum { DGV_FP64 = 0, DGV_FP32_COMP = 1 };                 /* precision       */
-enum { DGV_KERNEL_AUTO = 0, DGV_KERNEL_PAIRBLOCK = 1, DGV_KERNEL_RECORD = 2 };
+enum { DGV_KERNEL_AUTO = 0, DGV_KERNEL_PAIRBLOCK = 1, DGV_KERNEL_RECORD = 2, DGV_KERNEL_MFMA = 3 };
 
 typedef struct dgv_ctx dgv_ctx;
 

@@ -41,13 +41,14 @@ def test_geometry_matches_oracle(case):
     assert np.array_equal(du, G.nodes_dui) and np.array_equal(dw, G.nodes_dwi)
 
 
-def test_pairblock_kernel(case):
+@pytest.mark.parametrize("kernel", ["PAIRBLOCK", "MFMA"])
+def test_pairblock_kernels(case, kernel):
     import dgv0d3v_direct_collision_amd as dgv
     G, A, d, f, ref = case
-    d.set_kernel(dgv.DGV_KERNEL_PAIRBLOCK if np.prod(G.s) in (1, 8, 27) else dgv.DGV_KERNEL_AUTO)
+    d.set_kernel(getattr(dgv, "DGV_KERNEL_" + kernel))
     q = d.EvalCollisionPeriodicAPlus_DGV(f)
-    assert rel_l2(q, ref) <= TOL, rel_l2(q, ref)
     d.set_kernel(dgv.DGV_KERNEL_AUTO)
+    assert rel_l2(q, ref) <= TOL, rel_l2(q, ref)
 
 
 def test_record_kernel(case):
