@@ -8,7 +8,8 @@ host-side mirror of the reference's procedures (see ``collision.DGV``).
 from ._lib import (DGV_FP32_COMP, DGV_FP64, DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK,  # noqa: F401
                    DGV_KERNEL_MFMA, DGV_KERNEL_RECORD, DGVError, LIB_PATH, load)
 from .collision import DGV, DGVBatch, agen_pair, chunk_name, grid_desc, grid_geometry  # noqa: F401
+from .surrogate import Surrogate  # noqa: F401
 
-__all__ = ["DGV", "DGVBatch", "DGVError", "agen_pair", "grid_geometry", "chunk_name", "grid_desc", "load", "LIB_PATH",
+__all__ = ["DGV", "DGVBatch", "Surrogate", "DGVError", "agen_pair", "grid_geometry", "chunk_name", "grid_desc", "load", "LIB_PATH",
            "DGV_FP64", "DGV_FP32_COMP", "DGV_KERNEL_AUTO", "DGV_KERNEL_PAIRBLOCK",
            "DGV_KERNEL_RECORD", "DGV_KERNEL_MFMA"]

@@ -46,6 +46,7 @@ _dbl = C.c_double
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); every symbol of include/dgv.h
 SIGNATURES = {
@@ -92,6 +93,20 @@ SIGNATURES = {
     "dgv_batch_rk_run": (C.c_int, [_vp, _dbl, _dbl, _i32, _i64, C.c_int]),
     "dgv_batch_autotune": (C.c_int, [_vp, _ip, _dp]),
     "dgv_batch_device_buffers": (C.c_int, [_vp, _pp, _pp]),
+    "dgv_batch_set_probe_cells": (C.c_int, [_vp, _ip, _i32]),
+    "dgv_batch_eval_q_probe": (C.c_int, [_vp]),
+    "dgv_sur_create": (C.c_int, [_vp, _ip, _i32, _i32, C.c_uint64, _pp]),
+    "dgv_sur_destroy": (None, [_vp]),
+    "dgv_sur_set_scales": (C.c_int, [_vp, _dbl, _dbl]),
+    "dgv_sur_layer_shape": (C.c_int, [_vp, _i32, _ip, _ip]),
+    "dgv_sur_get_layer": (C.c_int, [_vp, _i32, _fp, _fp]),
+    "dgv_sur_set_layer": (C.c_int, [_vp, _i32, _fp, _fp]),
+    "dgv_sur_get_grad": (C.c_int, [_vp, _i32, _fp, _fp]),
+    "dgv_sur_train_step": (C.c_int, [_vp, _vp, _vp, _i64, _i32, _dbl, _dp]),
+    "dgv_sur_predict": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i32]),
+    "dgv_sur_set_probe_cells": (C.c_int, [_vp, _ip, _i32]),
+    "dgv_sur_eval": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _dbl, _dp, _ip, _lp]),
+    "dgv_sur_time_predict": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _dp]),
     "dgv_ctx_synchronize": (C.c_int, [_vp]),
     "dgv_time_eval_q_dev": (C.c_int, [_vp, _vp, _vp, _i32, _dp, _dp]),
     "dgv_batch_time_eval": (C.c_int, [_vp, _i32, _dp, _dp]),
@@ -133,3 +148,7 @@ def dptr(a: np.ndarray):
 
 def iptr(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)

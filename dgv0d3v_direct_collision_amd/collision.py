@@ -398,6 +398,15 @@ class DGVBatch:
     def rk_run(self, dt: float, nsteps: int, rkmts: int = 2, coef: float = 1.0, use_graph=True):
         self._chk(self._L.dgv_batch_rk_run(self._b, dt, coef, rkmts, nsteps, int(use_graph)))
 
+    def set_probe_cells(self, cells):
+        """Cells (0-based) evaluated by ``eval_q_probe``."""
+        c = np.ascontiguousarray(cells, np.int32)
+        self._chk(self._L.dgv_batch_set_probe_cells(self._b, iptr(c), len(c)))
+
+    def eval_q_probe(self):
+        """Q of every instance on the probe cells only."""
+        self._chk(self._L.dgv_batch_eval_q_probe(self._b))
+
     def autotune(self):
         best = C.c_int32()
         ms = C.c_double()

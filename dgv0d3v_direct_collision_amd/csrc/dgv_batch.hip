@@ -46,6 +46,22 @@ struct BatchItem {
   int64_t prow;                // first partial row (64 rows per item)
 };
 
+// One work decomposition: the items of the queue and the fixed-order reduction
+// of their partial rows.  A batch keeps the plan over all cells and, on
+// request, a probe plan over a chosen set of cells.
+struct BatchPlan {
+  int32_t tile = 0;
+  std::vector<BatchItem> items;
+  BatchItem* d_items = nullptr;
+  int64_t nrows = 0;
+  double* d_partial = nullptr;
+  // per (grid, cell, chunk): the partial rows in item order
+  int64_t nred = 0;
+  int32_t* d_red_rows = nullptr;
+  int64_t* d_red_ptr = nullptr;
+  int32_t* d_red_key = nullptr;    // (grid, cell, chunk) per reduction task
+};
+
 }  // namespace dgv
 
 struct dgv_batch {
@@ -64,17 +80,10 @@ struct dgv_batch {
   std::vector<std::vector<int64_t>> cell_ptr;  // per grid: ncells+1 offsets into blist
   std::vector<float*> d_pairs32;
   dgv::BatchGridDev* d_grids = nullptr;
-  int32_t tile = 0, waves_per_cu = 0;
-  std::vector<dgv::BatchItem> items;
-  dgv::BatchItem* d_items = nullptr;
-  int64_t nrows = 0;
-  double* d_partial = nullptr;
-  // reduction plan: per (grid, cell, chunk) the partial rows in item order
-  std::vector<int64_t> red_ptr;
-  int32_t* d_red_rows = nullptr;
-  int64_t* d_red_ptr = nullptr;
-  int32_t* d_red_key = nullptr;    // (grid, cell, chunk) per reduction task, packed
-  int64_t nred = 0;
+  int32_t waves_per_cu = 0;
+  dgv::BatchPlan plan;             // all cells
+  dgv::BatchPlan probe;            // the probe cells (empty until set)
+  std::vector<int32_t> probe_cells;
   int* d_counter = nullptr;
   int ncu = 256;
   hipStream_t stream = nullptr;
@@ -307,19 +316,22 @@ __global__ void k_blincomb(int64_t n, double* __restrict__ out, BLinComb c) {
   out[i] = s;
 }
 
-static void destroy_plan(dgv_batch* b) {
-  hipFree(b->d_items); hipFree(b->d_partial); hipFree(b->d_red_rows); hipFree(b->d_red_ptr); hipFree(b->d_red_key);
-  b->d_items = nullptr; b->d_partial = nullptr; b->d_red_rows = nullptr; b->d_red_ptr = nullptr; b->d_red_key = nullptr;
-  b->items.clear();
-  if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+static void destroy_plan(dgv_batch* b, BatchPlan& P) {
+  hipFree(P.d_items); hipFree(P.d_partial); hipFree(P.d_red_rows); hipFree(P.d_red_ptr); hipFree(P.d_red_key);
+  P.d_items = nullptr; P.d_partial = nullptr; P.d_red_rows = nullptr; P.d_red_ptr = nullptr; P.d_red_key = nullptr;
+  P.items.clear();
+  P.nrows = 0;
+  P.nred = 0;
+  if (&P == &b->plan && b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
 }
 
 // Per cell and grid: blocks whose box contains the cell, cut into segments of
-// about `tile` pairs; items = segments x chunks of 64 instances.
-static int build_batch_plan(dgv_batch* b, int32_t tile) {
-  destroy_plan(b);
+// about `tile` pairs; items = segments x chunks of 64 instances.  With `cells`
+// given, only those cells (of every grid) are planned.
+static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std::vector<int32_t>* cells = nullptr) {
+  destroy_plan(b, P);
   if (tile <= 0) tile = 1 << 20;
-  b->tile = tile;
+  P.tile = tile;
   std::vector<BatchItem> items;
   struct Task { int32_t g, c, vc; std::vector<int32_t> rows; };
   std::vector<int32_t> key;
@@ -345,7 +357,14 @@ static int build_batch_plan(dgv_batch* b, int32_t tile) {
         }
     }
     const int nchunk = b->Bpad[gi] / 64;
-    for (int64_t c = 0; c < ctx->g.ncells; ++c) {
+    std::vector<int64_t> todo;
+    if (cells) {
+      for (int32_t c : *cells)
+        if (c >= 0 && c < ctx->g.ncells) todo.push_back(c);
+    } else {
+      for (int64_t c = 0; c < ctx->g.ncells; ++c) todo.push_back(c);
+    }
+    for (int64_t c : todo) {
       std::vector<std::pair<int32_t, int32_t>> segs;
       int64_t s0 = cp[c], acc = 0;
       for (int64_t k = cp[c]; k < cp[c + 1]; ++k) {
@@ -373,26 +392,27 @@ static int build_batch_plan(dgv_batch* b, int32_t tile) {
   }
   const int D = b->grids[0]->g.D;
   // keep the partial sums within 4 GiB: coarser segments otherwise
-  if ((int64_t)items.size() * 64 * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30)) return build_batch_plan(b, tile * 2);
-  b->items.swap(items);
-  b->nrows = (int64_t)b->items.size() * 64;
-  b->nred = (int64_t)rptr.size() - 1;
-  BHIP(b, hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(b->items.size(), 1)));
-  BHIP(b, hipMemcpy(b->d_items, b->items.data(), sizeof(BatchItem) * b->items.size(), hipMemcpyHostToDevice));
-  if (hipMalloc(&b->d_partial, sizeof(double) * std::max<int64_t>(b->nrows * D, 1)) != hipSuccess)
+  if ((int64_t)items.size() * 64 * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30))
+    return build_batch_plan(b, P, tile * 2, cells);
+  P.items.swap(items);
+  P.nrows = (int64_t)P.items.size() * 64;
+  P.nred = (int64_t)rptr.size() - 1;
+  BHIP(b, hipMalloc(&P.d_items, sizeof(BatchItem) * std::max<size_t>(P.items.size(), 1)));
+  BHIP(b, hipMemcpy(P.d_items, P.items.data(), sizeof(BatchItem) * P.items.size(), hipMemcpyHostToDevice));
+  if (hipMalloc(&P.d_partial, sizeof(double) * std::max<int64_t>(P.nrows * D, 1)) != hipSuccess)
     return berr(b, DGV_ERR_NOMEM, "hipMalloc(batch partial sums) failed");
-  BHIP(b, hipMalloc(&b->d_red_rows, sizeof(int32_t) * std::max<size_t>(rrows.size(), 1)));
-  BHIP(b, hipMemcpy(b->d_red_rows, rrows.data(), sizeof(int32_t) * rrows.size(), hipMemcpyHostToDevice));
-  BHIP(b, hipMalloc(&b->d_red_ptr, sizeof(int64_t) * rptr.size()));
-  BHIP(b, hipMemcpy(b->d_red_ptr, rptr.data(), sizeof(int64_t) * rptr.size(), hipMemcpyHostToDevice));
-  BHIP(b, hipMalloc(&b->d_red_key, sizeof(int32_t) * std::max<size_t>(key.size(), 1)));
-  BHIP(b, hipMemcpy(b->d_red_key, key.data(), sizeof(int32_t) * key.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMalloc(&P.d_red_rows, sizeof(int32_t) * std::max<size_t>(rrows.size(), 1)));
+  BHIP(b, hipMemcpy(P.d_red_rows, rrows.data(), sizeof(int32_t) * rrows.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMalloc(&P.d_red_ptr, sizeof(int64_t) * rptr.size()));
+  BHIP(b, hipMemcpy(P.d_red_ptr, rptr.data(), sizeof(int64_t) * rptr.size(), hipMemcpyHostToDevice));
+  BHIP(b, hipMalloc(&P.d_red_key, sizeof(int32_t) * std::max<size_t>(key.size(), 1)));
+  BHIP(b, hipMemcpy(P.d_red_key, key.data(), sizeof(int32_t) * key.size(), hipMemcpyHostToDevice));
   return DGV_OK;
 }
 
-static int batch_eval_dev(dgv_batch* b, const double* d_f, double* d_q, double scale) {
+static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, double* d_q, double scale) {
   const int D = b->grids[0]->g.D;
-  const int nitems = (int)b->items.size();
+  const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
   if (b->precision == DGV_FP32_COMP)
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
@@ -402,27 +422,27 @@ static int batch_eval_dev(dgv_batch* b, const double* d_f, double* d_q, double s
   if (nitems > 0) {
     if (D == 27) {
       if (b->precision == DGV_FP32_COMP)
-        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, b->d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, b->d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
     } else {
       if (b->precision == DGV_FP32_COMP)
-        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, b->d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), dim3(64), 0, st, b->d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, b->d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
     }
   }
-  if (b->nred > 0) {
+  if (P.nred > 0) {
     if (D == 27)
-      hipLaunchKernelGGL(k_batch_reduce<27>, dim3((unsigned)b->nred), dim3(64), 0, st, b->nred, b->d_red_key,
-                         b->d_red_ptr, b->d_red_rows, b->d_grids, b->d_partial, scale, d_q);
+      hipLaunchKernelGGL(k_batch_reduce<27>, dim3((unsigned)P.nred), dim3(64), 0, st, P.nred, P.d_red_key,
+                         P.d_red_ptr, P.d_red_rows, b->d_grids, P.d_partial, scale, d_q);
     else
-      hipLaunchKernelGGL(k_batch_reduce<8>, dim3((unsigned)b->nred), dim3(64), 0, st, b->nred, b->d_red_key,
-                         b->d_red_ptr, b->d_red_rows, b->d_grids, b->d_partial, scale, d_q);
+      hipLaunchKernelGGL(k_batch_reduce<8>, dim3((unsigned)P.nred), dim3(64), 0, st, P.nred, P.d_red_key,
+                         P.d_red_ptr, P.d_red_rows, b->d_grids, P.d_partial, scale, d_q);
   }
   BHIP(b, hipGetLastError());
   return DGV_OK;
@@ -445,7 +465,7 @@ static int batch_rk_step(dgv_batch* b, double dt, double coef, int rkmts) {
   double *f = b->d_f, *k1 = b->d_tmp[0], *k2 = b->d_tmp[1], *k3 = b->d_tmp[2], *tmp = b->d_tmp[3];
   const double ct = coef * dt;
   int rc;
-#define BSOP(in, out) do { rc = batch_eval_dev(b, in, out, ct); if (rc) return rc; } while (0)
+#define BSOP(in, out) do { rc = batch_eval_dev(b, b->plan, in, out, ct); if (rc) return rc; } while (0)
   switch (rkmts) {
     case 1: {
       BSOP(f, k1);
@@ -595,7 +615,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
       hipMemcpy(b->d_grids, gd.data(), sizeof(BatchGridDev) * ngrids, hipMemcpyHostToDevice))
     return fail(DGV_ERR_HIP, "grid table");
   if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(DGV_ERR_HIP, "coefficient conversion");
-  int rc = build_batch_plan(b, 0);
+  int rc = build_batch_plan(b, b->plan, 0);
   if (rc) { std::string m = b->err; dgv_batch_destroy(b); set_global_error(m); return rc; }
   *out = b;
   return DGV_OK;
@@ -605,7 +625,8 @@ void dgv_batch_destroy(dgv_batch* b) {
   if (!b) return;
   if (!b->grids.empty()) hipSetDevice(b->grids[0]->device);
   if (b->stream) hipStreamSynchronize(b->stream);
-  destroy_plan(b);
+  destroy_plan(b, b->plan);
+  destroy_plan(b, b->probe);
   hipFree(b->d_f); hipFree(b->d_q); hipFree(b->d_f32); hipFree(b->d_counter); hipFree(b->d_grids);
   for (int i = 0; i < 4; ++i) hipFree(b->d_tmp[i]);
   for (auto p : b->d_blist) hipFree(p);
@@ -653,7 +674,31 @@ int dgv_batch_get_q(dgv_batch* b, int32_t instance, double* q, int64_t n) {
 int dgv_batch_eval_q(dgv_batch* b) {
   if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
   BHIP(b, hipSetDevice(b->grids[0]->device));
-  int rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+  int rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0);
+  if (rc) return rc;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_set_probe_cells(dgv_batch* b, const int32_t* cells, int32_t ncells) {
+  if (!b || ncells < 0 || (ncells > 0 && !cells)) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->probe_cells.assign(cells, cells + ncells);
+  std::sort(b->probe_cells.begin(), b->probe_cells.end());
+  b->probe_cells.erase(std::unique(b->probe_cells.begin(), b->probe_cells.end()), b->probe_cells.end());
+  for (int32_t c : b->probe_cells)
+    for (const dgv_ctx* g : b->grids)
+      if (c < 0 || c >= g->g.ncells) return berr(b, DGV_ERR_INDEX, "probe cell outside a grid");
+  if (ncells == 0) { destroy_plan(b, b->probe); return DGV_OK; }
+  return build_batch_plan(b, b->probe, b->plan.tile, &b->probe_cells);
+}
+
+int dgv_batch_eval_q_probe(dgv_batch* b) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  if (b->probe.items.empty()) return berr(b, DGV_ERR_STATE, "no probe cells set");
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  int rc = batch_eval_dev(b, b->probe, b->d_f, b->d_q, 1.0);
   if (rc) return rc;
   BHIP(b, hipStreamSynchronize(b->stream));
   return DGV_OK;
@@ -671,7 +716,7 @@ int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_
     BHIP(b, hipStreamSynchronize(b->stream));
     return DGV_OK;
   }
-  const bool same = b->graph && b->graph_key[0] == rkmts && b->graph_key[1] == b->tile && b->graph_dt == dt &&
+  const bool same = b->graph && b->graph_key[0] == rkmts && b->graph_key[1] == b->plan.tile && b->graph_dt == dt &&
                     b->graph_coef == coef;
   if (!same) {
     if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
@@ -685,7 +730,7 @@ int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_
     hipGraphDestroy(graph);
     if (e != hipSuccess) return berr(b, DGV_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     b->graph_key[0] = rkmts;
-    b->graph_key[1] = b->tile;
+    b->graph_key[1] = b->plan.tile;
     b->graph_dt = dt;
     b->graph_coef = coef;
   }
@@ -699,7 +744,7 @@ int dgv_batch_time_eval(dgv_batch* b, int32_t niter, double* ms_per_call, double
   BHIP(b, hipSetDevice(b->grids[0]->device));
   BHIP(b, hipEventRecord(b->ev[0], b->stream));
   for (int i = 0; i < niter; ++i) {
-    int rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+    int rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0);
     if (rc) return rc;
   }
   BHIP(b, hipEventRecord(b->ev[1], b->stream));
@@ -719,12 +764,12 @@ int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
   int32_t bt = 0, bw = 0;
   double bms = 1e300;
   for (int32_t t : tiles) {
-    int rc = build_batch_plan(b, t);
+    int rc = build_batch_plan(b, b->plan, t);
     if (rc) return rc;
     for (int32_t w : wpcs) {
       b->waves_per_cu = w;
       double ms;
-      rc = batch_eval_dev(b, b->d_f, b->d_q, 1.0);
+      rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0);
       if (rc) return rc;
       rc = dgv_batch_time_eval(b, 2, &ms, nullptr);
       if (rc) return rc;
@@ -732,8 +777,9 @@ int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
     }
   }
   b->waves_per_cu = bw;
-  int rc = build_batch_plan(b, bt);
+  int rc = build_batch_plan(b, b->plan, bt);
   if (rc) return rc;
+  if (!b->probe_cells.empty() && (rc = build_batch_plan(b, b->probe, bt, &b->probe_cells))) return rc;
   if (best) *best = bt;
   if (best_ms) *best_ms = bms;
   return DGV_OK;

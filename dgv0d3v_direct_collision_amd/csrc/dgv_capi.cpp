@@ -34,7 +34,7 @@ using namespace dgv;
 
 // Cholesky factor of the Gram matrix G_jk = sum_i w_i psi_j psi_k
 // (psi = 1,u,v,w,|u|^2), accumulated in long double.
-static int gram_cholesky(const Grid& g, double L[25]) {
+int dgv::gram_cholesky(const Grid& g, double L[25]) {
   long double G[5][5] = {};
   for (int64_t i = 0; i < g.N; ++i) {
     const long double u = g.u[i], v = g.v[i], w = g.w[i];

@@ -170,6 +170,9 @@ int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);
 int device_moments(dgv_ctx* ctx, const double* d_f, double out[5]);
 int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]);
 int device_project(dgv_ctx* ctx, double* d_q);
+// Cholesky factor (row-major, lower) of the Gram matrix of psi = 1,u,v,w,|u|^2
+// in the w-weighted inner product (dgv_capi.cpp)
+int gram_cholesky(const Grid& g, double L[25]);
 
 // generator (dgv_agen.hip)
 int generate_pairblock(dgv_ctx* ctx, const dgv_agen_params& p);

@@ -1,0 +1,924 @@
+// dgv_learn.hip -- learned surrogate of the collision operator Q(f).
+//
+// An MLP maps a VDF (reference node order, scaled by 1/f_scale) to Q(f)/q_scale:
+//   h_0 = x,  z_l = h_l W_l^T + b_l,  h_{l+1} = GELU(z_l)  (l < L-1),  y = z_{L-1}.
+// Weights and activations are BF16 on the matrix cores (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation); master weights, Adam moments, biases and pre-activations
+// are FP32.  The output is scaled back to FP64 and projected onto the
+// complement of the collision invariants psi = 1, u, v, w, |u|^2 in the
+// w-weighted inner product, so the surrogate's Q conserves density, momentum
+// and energy to FP64 rounding.  The direct evaluator is the surrogate's judge:
+// on a set of probe cells the batched FP64 engine gives each row's residual,
+// and rows whose residual exceeds the tolerance receive the direct Q.
+//
+// Every matrix product has the form C[m][n] = sum_k A[m][k] B[n][k] with both
+// operands K-contiguous, so one kernel serves all of them:
+//   forward       z  = h  . Wf^T     A = h   [B][in],   B = Wf [out][in]
+//   input grad    dh = dz . Wb^T     A = dz  [B][out],  B = Wb [in][out]
+//   weight grad   dW = dz^T . h      A = dzT [out][B],  B = hT [in][B]
+// The epilogues write each activation in both orientations (h and hT, dz and
+// dzT) and the Adam kernel writes both weight orientations (Wf and Wb).
+// Widths are padded to multiples of 128 with zero weights, which stay zero.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "dgv_internal.h"
+
+namespace dgv {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU with the exact normal CDF and its derivative
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ---------------------------------------------------------------------------------
+// C[s][m][n] = sum_{k in chunk s} A[m][k] B[n][k].  Block tile 128 x 128, k step
+// 64, four waves each owning a 64 x 64 quadrant as 4 x 4 MFMA tiles of 16 x 16.
+// Requirements (checked by the host): M, N multiples of 128, K and kchunk
+// multiples of 64, lda and ldb multiples of 8.
+constexpr int GT_M = 128, GT_N = 128, GT_K = 64, G_SLD = GT_K + 8;
+
+__global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, int kchunk, const uint16_t* __restrict__ A,
+                                                 int64_t lda, const uint16_t* __restrict__ B, int64_t ldb,
+                                                 float* __restrict__ C) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[GT_M * G_SLD];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[GT_N * G_SLD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n0 = blockIdx.x * GT_N, m0 = blockIdx.y * GT_M;
+  const int kb0 = blockIdx.z * kchunk;
+  const int kend = min(K, kb0 + kchunk);
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  // thread t moves the 16-byte vectors t + 256 i (row v / 8, columns 8 (v % 8) ..) of both tiles
+  const int lr = tid >> 3, lc = (tid & 7) * 8;
+  const uint16_t* Ag = A + (int64_t)(m0 + lr) * lda + lc;
+  const uint16_t* Bg = B + (int64_t)(n0 + lr) * ldb + lc;
+  uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define GEMM_GLOAD(kb)                                                      \
+  do {                                                                      \
+    ra0 = *reinterpret_cast<const uint4*>(Ag + (kb));                       \
+    ra1 = *reinterpret_cast<const uint4*>(Ag + 32 * lda + (kb));            \
+    ra2 = *reinterpret_cast<const uint4*>(Ag + 64 * lda + (kb));            \
+    ra3 = *reinterpret_cast<const uint4*>(Ag + 96 * lda + (kb));            \
+    rb0 = *reinterpret_cast<const uint4*>(Bg + (kb));                       \
+    rb1 = *reinterpret_cast<const uint4*>(Bg + 32 * ldb + (kb));            \
+    rb2 = *reinterpret_cast<const uint4*>(Bg + 64 * ldb + (kb));            \
+    rb3 = *reinterpret_cast<const uint4*>(Bg + 96 * ldb + (kb));            \
+  } while (0)
+  if (kb0 < kend) GEMM_GLOAD(kb0);
+  for (int kb = kb0; kb < kend; kb += GT_K) {
+    __syncthreads();
+    *reinterpret_cast<uint4*>(sA + lr * G_SLD + lc) = ra0;
+    *reinterpret_cast<uint4*>(sA + (lr + 32) * G_SLD + lc) = ra1;
+    *reinterpret_cast<uint4*>(sA + (lr + 64) * G_SLD + lc) = ra2;
+    *reinterpret_cast<uint4*>(sA + (lr + 96) * G_SLD + lc) = ra3;
+    *reinterpret_cast<uint4*>(sB + lr * G_SLD + lc) = rb0;
+    *reinterpret_cast<uint4*>(sB + (lr + 32) * G_SLD + lc) = rb1;
+    *reinterpret_cast<uint4*>(sB + (lr + 64) * G_SLD + lc) = rb2;
+    *reinterpret_cast<uint4*>(sB + (lr + 96) * G_SLD + lc) = rb3;
+    __syncthreads();
+    if (kb + GT_K < kend) GEMM_GLOAD(kb + GT_K);
+#pragma unroll
+    for (int kk = 0; kk < GT_K; kk += 32) {
+      bf16x8_t af[4], bv[4];
+      const int ko = kk + 8 * (lane >> 4);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        af[t] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(sA + (wm + 16 * t + (lane & 15)) * G_SLD + ko));
+        bv[t] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(sB + (wn + 16 * t + (lane & 15)) * G_SLD + ko));
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#undef GEMM_GLOAD
+  // output layout of the 16 x 16 FP32 tile: lane holds rows 4 (lane / 16) + e of column lane % 16
+  float* Cs = C + (int64_t)blockIdx.z * M * N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + e;
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        Cs[(int64_t)m * N + n] = acc[i][j][e];
+      }
+}
+
+// ---------------------------------------------------------------------------------
+// Epilogues over 64 x 64 tiles of an M x N result, 256 threads: thread (c, rg)
+// handles column n0 + c of rows m0 + rg + 4 i.  Transposed copies go through LDS.
+enum { EPI_GELU = 0, EPI_LINEAR = 1, EPI_BWD_GELU = 2 };
+
+struct EpiArgs {
+  const float* part;   // [S][M][N] partial sums
+  int S, M, N;
+  const float* bias;   // [N] (forward)
+  const float* zin;    // [M][N] pre-activation (backward)
+  float* zout;         // [M][N] pre-activation (GELU) or output (LINEAR)
+  uint16_t* h;         // [M][N] BF16: GELU(z) or dz
+  uint16_t* ht;        // [N][M] BF16 transposed copy
+  float* dbpart;       // [M / 64][N] column sums of dz (backward)
+};
+
+__global__ __launch_bounds__(256) void k_epilogue(EpiArgs a, int mode) {
+  __shared__ uint16_t tile[64][66];
+  __shared__ float cs[4][64];
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int64_t MN = (int64_t)a.M * a.N;
+  float colsum = 0.f;
+  for (int i = 0; i < 16; ++i) {
+    const int r = rg + 4 * i;
+    const int64_t idx = (int64_t)(m0 + r) * a.N + n0 + c;
+    float s = 0.f;
+    for (int k = 0; k < a.S; ++k) s += a.part[k * MN + idx];
+    if (mode == EPI_GELU) {
+      const float z = s + a.bias[n0 + c];
+      a.zout[idx] = z;
+      const uint16_t hb = f2bf(gelu(z));
+      a.h[idx] = hb;
+      tile[c][r] = hb;
+    } else if (mode == EPI_LINEAR) {
+      a.zout[idx] = s + a.bias[n0 + c];
+    } else {
+      const float dz = s * gelu_d(a.zin[idx]);
+      const uint16_t hb = f2bf(dz);
+      a.h[idx] = hb;
+      tile[c][r] = hb;
+      colsum += dz;
+    }
+  }
+  if (mode == EPI_LINEAR) return;
+  cs[rg][c] = colsum;
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int rr = rg + 4 * i;
+    a.ht[(int64_t)(n0 + rr) * a.M + m0 + c] = tile[rr][c];
+  }
+  if (mode == EPI_BWD_GELU && rg == 0)
+    a.dbpart[(int64_t)blockIdx.y * a.N + n0 + c] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
+}
+
+// Mean-squared-error gradient of the output: dz = gscale (y - t) on the valid
+// rows / columns, written as BF16 in both orientations, its column sums, and
+// the tile's sum of squared differences.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t, int M, int N,
+                                              int mvalid, int nvalid, float gscale, uint16_t* __restrict__ dz,
+                                              uint16_t* __restrict__ dzt, float* __restrict__ dbpart,
+                                              double* __restrict__ losspart) {
+  __shared__ uint16_t tile[64][66];
+  __shared__ float cs[4][64];
+  __shared__ double red[256];
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  float colsum = 0.f;
+  double sq = 0.0;
+  for (int i = 0; i < 16; ++i) {
+    const int r = rg + 4 * i;
+    const int64_t idx = (int64_t)(m0 + r) * N + n0 + c;
+    const bool ok = (m0 + r) < mvalid && (n0 + c) < nvalid;
+    const float d = ok ? y[idx] - t[idx] : 0.f;
+    sq += (double)d * (double)d;
+    const float g = gscale * d;
+    const uint16_t hb = f2bf(g);
+    dz[idx] = hb;
+    tile[c][r] = hb;
+    colsum += g;
+  }
+  cs[rg][c] = colsum;
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int rr = rg + 4 * i;
+    dzt[(int64_t)(n0 + rr) * M + m0 + c] = tile[rr][c];
+  }
+  if (rg == 0) dbpart[(int64_t)blockIdx.y * N + n0 + c] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) losspart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[n] = sum_t part[t][n], t in order
+__global__ void k_colsum(const float* __restrict__ part, int T, int N, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int t = 0; t < T; ++t) s += part[(int64_t)t * N + n];
+  out[n] = s;
+}
+
+// one block: out[0] = sum of part[0..n) in a fixed order, times scale
+__global__ __launch_bounds__(256) void k_sum_fixed(const double* __restrict__ part, int n, double scale,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;  // c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t)
+};
+
+__device__ __forceinline__ float adam_update(float g, float& w, float& m, float& v, const AdamArgs& h) {
+  m = h.b1 * m + (1.f - h.b1) * g;
+  v = h.b2 * v + (1.f - h.b2) * g * g;
+  w = w - h.lr * (m * h.c1) / (sqrtf(v * h.c2) + h.eps);
+  return w;
+}
+
+// Adam on a weight matrix [O][I] (64 x 64 tiles); writes the master weights,
+// the moments and the BF16 copies Wf [O][I] and Wb [I][O].
+__global__ __launch_bounds__(256) void k_adam_w(int O, int I, const float* __restrict__ G, float* __restrict__ W,
+                                                float* __restrict__ m, float* __restrict__ v,
+                                                uint16_t* __restrict__ Wf, uint16_t* __restrict__ Wb, AdamArgs h) {
+  __shared__ uint16_t tile[64][66];
+  const int i0 = blockIdx.x * 64, o0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  for (int k = 0; k < 16; ++k) {
+    const int r = rg + 4 * k;
+    const int64_t idx = (int64_t)(o0 + r) * I + i0 + c;
+    float w = W[idx], mm = m[idx], vv = v[idx];
+    adam_update(G[idx], w, mm, vv, h);
+    W[idx] = w;
+    m[idx] = mm;
+    v[idx] = vv;
+    const uint16_t hb = f2bf(w);
+    Wf[idx] = hb;
+    tile[c][r] = hb;
+  }
+  __syncthreads();
+  for (int k = 0; k < 16; ++k) {
+    const int rr = rg + 4 * k;
+    Wb[(int64_t)(i0 + rr) * O + o0 + c] = tile[rr][c];
+  }
+}
+
+__global__ void k_adam_vec(int n, const float* __restrict__ g, float* __restrict__ w, float* __restrict__ m,
+                           float* __restrict__ v, AdamArgs h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float ww = w[i], mm = m[i], vv = v[i];
+  adam_update(g[i], ww, mm, vv, h);
+  w[i] = ww;
+  m[i] = mm;
+  v[i] = vv;
+}
+
+// BF16 copies of master weights [O][I]: Wf [O][I] and Wb [I][O]
+__global__ __launch_bounds__(256) void k_weights_bf16(int O, int I, const float* __restrict__ W,
+                                                      uint16_t* __restrict__ Wf, uint16_t* __restrict__ Wb) {
+  __shared__ uint16_t tile[64][66];
+  const int i0 = blockIdx.x * 64, o0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  for (int k = 0; k < 16; ++k) {
+    const int r = rg + 4 * k;
+    const int64_t idx = (int64_t)(o0 + r) * I + i0 + c;
+    const uint16_t hb = f2bf(W[idx]);
+    Wf[idx] = hb;
+    tile[c][r] = hb;
+  }
+  __syncthreads();
+  for (int k = 0; k < 16; ++k) {
+    const int rr = rg + 4 * k;
+    Wb[(int64_t)(i0 + rr) * O + o0 + c] = tile[rr][c];
+  }
+}
+
+// Rows of f (FP64, [nrows][ld]) -> x = f / f_scale as BF16 [Mp][Kp] and [Kp][Mp],
+// zero outside nrows x N.
+__global__ __launch_bounds__(256) void k_rows_in(const double* __restrict__ F, int64_t ld, int nrows, int N,
+                                                 double inv_scale, int Mp, int Kp, uint16_t* __restrict__ X,
+                                                 uint16_t* __restrict__ Xt) {
+  __shared__ uint16_t tile[64][66];
+  const int k0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  for (int i = 0; i < 16; ++i) {
+    const int r = rg + 4 * i;
+    const int m = m0 + r, k = k0 + c;
+    const float x = (m < nrows && k < N) ? (float)(F[(int64_t)m * ld + k] * inv_scale) : 0.f;
+    const uint16_t hb = f2bf(x);
+    X[(int64_t)m * Kp + k] = hb;
+    tile[c][r] = hb;
+  }
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int rr = rg + 4 * i;
+    Xt[(int64_t)(k0 + rr) * Mp + m0 + c] = tile[rr][c];
+  }
+}
+
+// Rows of Q (FP64) -> targets q / q_scale in FP32 [Mp][Np], zero outside
+__global__ void k_rows_target(const double* __restrict__ Q, int64_t ld, int nrows, int N, double inv_scale, int Np,
+                              int64_t total, float* __restrict__ T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int m = (int)(i / Np), n = (int)(i % Np);
+  T[i] = (m < nrows && n < N) ? (float)(Q[(int64_t)m * ld + n] * inv_scale) : 0.f;
+}
+
+// Output rows: q = q_scale * y in FP64
+__global__ void k_rows_out(const float* __restrict__ Y, int Np, int nrows, int N, double scale,
+                           double* __restrict__ q, int64_t ld) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nrows * N) return;
+  const int m = (int)(i / N), n = (int)(i % N);
+  q[(int64_t)m * ld + n] = scale * (double)Y[(int64_t)m * Np + n];
+}
+
+struct Chol5 {
+  double L[25];
+};
+
+// FP64 projection of each row onto the complement of psi = 1,u,v,w,|u|^2 in the
+// w-weighted inner product: q -= sum_k lam_k psi_k with G lam = (sum_i w_i psi_k q_i).
+// One block per row; moments summed in a fixed order.
+__global__ __launch_bounds__(256) void k_project_rows(double* __restrict__ q, int64_t ld, int N,
+                                                      const double* __restrict__ u, const double* __restrict__ v,
+                                                      const double* __restrict__ w, const double* __restrict__ gw,
+                                                      Chol5 C) {
+  __shared__ double red[5][256];
+  __shared__ double lam[5];
+  double* row = q + (int64_t)blockIdx.x * ld;
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < N; i += 256) {
+    const double x = gw[i] * row[i];
+    const double uu = u[i], vv = v[i], ww = w[i];
+    s[0] += x;
+    s[1] += x * uu;
+    s[2] += x * vv;
+    s[3] += x * ww;
+    s[4] += x * (uu * uu + vv * vv + ww * ww);
+  }
+  for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double y[5], x[5];
+    for (int i = 0; i < 5; ++i) {
+      double a = red[i][0];
+      for (int k = 0; k < i; ++k) a -= C.L[i * 5 + k] * y[k];
+      y[i] = a / C.L[i * 5 + i];
+    }
+    for (int i = 4; i >= 0; --i) {
+      double a = y[i];
+      for (int k = i + 1; k < 5; ++k) a -= C.L[k * 5 + i] * x[k];
+      x[i] = a / C.L[i * 5 + i];
+    }
+    for (int k = 0; k < 5; ++k) lam[k] = x[k];
+  }
+  __syncthreads();
+  const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3], l4 = lam[4];
+  for (int i = threadIdx.x; i < N; i += 256) {
+    const double uu = u[i], vv = v[i], ww = w[i];
+    row[i] = row[i] - (l0 + l1 * uu + l2 * vv + l3 * ww + l4 * (uu * uu + vv * vv + ww * ww));
+  }
+}
+
+// rows [nrows][ld] -> packed batch layout P[node * stride + r] (64 x 64 tiles)
+__global__ __launch_bounds__(256) void k_rows_to_packed(const double* __restrict__ F, int64_t ld, int nrows, int N,
+                                                        double* __restrict__ P, int64_t stride) {
+  __shared__ double tile[64][65];
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  for (int i = 0; i < 16; ++i) {
+    const int r = rg + 4 * i;
+    const int m = m0 + r, n = n0 + c;
+    tile[c][r] = (m < nrows && n < N) ? F[(int64_t)m * ld + n] : 0.0;
+  }
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int rr = rg + 4 * i;
+    const int n = n0 + rr, m = m0 + c;
+    if (n < N && m < nrows) P[(int64_t)n * stride + m] = tile[rr][c];
+  }
+}
+
+// Residual of each row over the probe nodes: ||q_sur - q_dir|| / ||q_dir||,
+// q_dir in the packed layout.  One block per row, fixed-order sums.
+__global__ __launch_bounds__(256) void k_probe_residual(const double* __restrict__ q, int64_t ld,
+                                                        const double* __restrict__ P, int64_t stride,
+                                                        const int32_t* __restrict__ nodes, int nn,
+                                                        double* __restrict__ res) {
+  __shared__ double r0[256], r1[256];
+  const int m = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nn; i += 256) {
+    const int64_t node = nodes[i];
+    const double d = P[node * stride + m];
+    const double e = q[(int64_t)m * ld + node] - d;
+    a += e * e;
+    b += d * d;
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      r0[threadIdx.x] += r0[threadIdx.x + st];
+      r1[threadIdx.x] += r1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[m] = r1[0] > 0.0 ? sqrt(r0[0] / r1[0]) : (r0[0] > 0.0 ? INFINITY : 0.0);
+}
+
+}  // namespace dgv
+
+// ===================================================================================
+// host side
+
+namespace {
+
+using namespace dgv;
+
+int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
+
+struct SurLayer {
+  int in = 0, out = 0, inp = 0, outp = 0;
+  float *W = nullptr, *mW = nullptr, *vW = nullptr, *G = nullptr;      // [outp][inp] FP32
+  float *b = nullptr, *mb = nullptr, *vb = nullptr, *gb = nullptr;     // [outp]
+  uint16_t *Wf = nullptr, *Wb = nullptr;                               // BF16 [outp][inp], [inp][outp]
+};
+
+struct Gemm {
+  int M, N, K, S, kchunk;
+};
+
+Gemm plan_gemm(int M, int N, int K, bool allow_split) {
+  Gemm g{M, N, K, 1, K};
+  if (allow_split) {
+    const int tiles = (M / GT_M) * (N / GT_N);
+    const int kt = K / GT_K;
+    int S = std::max(1, std::min(kt, (1024 + tiles - 1) / tiles));
+    g.kchunk = ((kt + S - 1) / S) * GT_K;
+    g.S = (K + g.kchunk - 1) / g.kchunk;
+  }
+  return g;
+}
+
+}  // namespace
+
+struct dgv_surrogate {
+  dgv_ctx* ctx = nullptr;
+  int L = 0, batch = 0, Bp = 0;
+  std::vector<SurLayer> layers;
+  std::vector<uint16_t*> H, Ht;  // H[l]: input of layer l, BF16 [Bp][inp_l] and [inp_l][Bp]
+  std::vector<float*> Z;         // Z[l], l < L-1: pre-activation [Bp][outp_l]
+  float* Y = nullptr;            // [Bp][outp_{L-1}]
+  float* T = nullptr;
+  uint16_t* dZ[2] = {nullptr, nullptr};
+  uint16_t* dZt[2] = {nullptr, nullptr};
+  float* part = nullptr;
+  int64_t part_elems = 0;
+  float* dbpart = nullptr;
+  double* losspart = nullptr;
+  double* d_loss = nullptr;
+  int64_t step = 0;
+  double f_scale = 1.0, q_scale = 1.0;
+  Chol5 chol{};
+  // judge / fallback
+  dgv_batch* judge = nullptr;
+  int32_t* d_probe_nodes = nullptr;
+  int nprobe_nodes = 0;
+  double* d_res = nullptr;
+  std::vector<int32_t> probe_cells;
+};
+
+namespace {
+
+int serr(dgv_surrogate* s, int code, const std::string& m) {
+  return set_error(s ? s->ctx : nullptr, code, m);
+}
+#define SHIP(s, call)                                                                                    \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return serr((s), DGV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int launch_gemm(dgv_surrogate* s, const Gemm& g, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb,
+                float* C) {
+  if (g.M % GT_M || g.N % GT_N || g.K % GT_K || g.kchunk % GT_K || lda % 8 || ldb % 8 || lda < g.K || ldb < g.K)
+    return serr(s, DGV_ERR_ARG, "matrix product shape does not fit the tiling");
+  hipLaunchKernelGGL(k_gemm_nt, dim3(g.N / GT_N, g.M / GT_M, g.S), dim3(256), 0, s->ctx->stream, g.M, g.N, g.K,
+                     g.kchunk, A, lda, B, ldb, C);
+  return DGV_OK;
+}
+
+int forward(dgv_surrogate* s) {
+  hipStream_t st = s->ctx->stream;
+  for (int l = 0; l < s->L; ++l) {
+    const SurLayer& ly = s->layers[l];
+    const Gemm g = plan_gemm(s->Bp, ly.outp, ly.inp, true);
+    int rc = launch_gemm(s, g, s->H[l], ly.inp, ly.Wf, ly.inp, s->part);
+    if (rc) return rc;
+    EpiArgs a{};
+    a.part = s->part;
+    a.S = g.S;
+    a.M = s->Bp;
+    a.N = ly.outp;
+    a.bias = ly.b;
+    if (l + 1 < s->L) {
+      a.zout = s->Z[l];
+      a.h = s->H[l + 1];
+      a.ht = s->Ht[l + 1];
+      hipLaunchKernelGGL(k_epilogue, dim3(ly.outp / 64, s->Bp / 64), dim3(256), 0, st, a, (int)EPI_GELU);
+    } else {
+      a.zout = s->Y;
+      hipLaunchKernelGGL(k_epilogue, dim3(ly.outp / 64, s->Bp / 64), dim3(256), 0, st, a, (int)EPI_LINEAR);
+    }
+  }
+  SHIP(s, hipGetLastError());
+  return DGV_OK;
+}
+
+int load_rows(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows) {
+  const SurLayer& l0 = s->layers[0];
+  hipLaunchKernelGGL(k_rows_in, dim3(l0.inp / 64, s->Bp / 64), dim3(256), 0, s->ctx->stream, d_f, ld, nrows, l0.in,
+                     1.0 / s->f_scale, s->Bp, l0.inp, s->H[0], s->Ht[0]);
+  SHIP(s, hipGetLastError());
+  return DGV_OK;
+}
+
+int predict_chunk(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows, double* d_q, int project) {
+  int rc = load_rows(s, d_f, ld, nrows);
+  if (rc) return rc;
+  if ((rc = forward(s))) return rc;
+  const SurLayer& lo = s->layers[s->L - 1];
+  const int64_t n = (int64_t)nrows * lo.out;
+  hipLaunchKernelGGL(k_rows_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->ctx->stream, s->Y, lo.outp,
+                     nrows, lo.out, s->q_scale, d_q, ld);
+  if (project) {
+    const dgv_ctx* c = s->ctx;
+    hipLaunchKernelGGL(k_project_rows, dim3(nrows), dim3(256), 0, c->stream, d_q, ld, (int)c->g.N, c->d_u, c->d_v,
+                       c->d_w, c->d_gwts, s->chol);
+  }
+  SHIP(s, hipGetLastError());
+  return DGV_OK;
+}
+
+void free_surrogate(dgv_surrogate* s) {
+  for (SurLayer& l : s->layers) {
+    for (float* p : {l.W, l.mW, l.vW, l.G, l.b, l.mb, l.vb, l.gb}) hipFree(p);
+    hipFree(l.Wf);
+    hipFree(l.Wb);
+  }
+  for (auto p : s->H) hipFree(p);
+  for (auto p : s->Ht) hipFree(p);
+  for (auto p : s->Z) hipFree(p);
+  hipFree(s->Y); hipFree(s->T);
+  for (int i = 0; i < 2; ++i) { hipFree(s->dZ[i]); hipFree(s->dZt[i]); }
+  hipFree(s->part); hipFree(s->dbpart); hipFree(s->losspart); hipFree(s->d_loss);
+  hipFree(s->d_probe_nodes); hipFree(s->d_res);
+  if (s->judge) dgv_batch_destroy(s->judge);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t batch, uint64_t seed,
+                   dgv_surrogate** out) {
+  if (!ctx || !widths || !out || nlayers < 1 || batch < 1) { set_global_error("dgv_sur_create: bad argument"); return DGV_ERR_ARG; }
+  *out = nullptr;
+  if (widths[0] != ctx->g.N || widths[nlayers] != ctx->g.N)
+    return set_error(ctx, DGV_ERR_ARG, "the first and last widths must equal the number of nodes");
+  for (int l = 0; l <= nlayers; ++l)
+    if (widths[l] < 1) return set_error(ctx, DGV_ERR_ARG, "widths must be positive");
+  double L25[25];
+  if (gram_cholesky(ctx->g, L25)) return set_error(ctx, DGV_ERR_GRID, "Gram matrix of the invariants is singular");
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  dgv_surrogate* s = new dgv_surrogate();
+  s->ctx = ctx;
+  s->L = nlayers;
+  s->batch = batch;
+  s->Bp = round_up(batch, 128);
+  std::memcpy(s->chol.L, L25, sizeof(L25));
+  auto fail = [&](int code, const std::string& m) { free_surrogate(s); delete s; return set_error(ctx, code, m); };
+  auto dalloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess; };
+  s->layers.resize(nlayers);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  int maxw = 0;
+  s->part_elems = 1;
+  for (int l = 0; l < nlayers; ++l) {
+    SurLayer& ly = s->layers[l];
+    ly.in = widths[l];
+    ly.out = widths[l + 1];
+    ly.inp = round_up(ly.in, 128);
+    ly.outp = round_up(ly.out, 128);
+    maxw = std::max({maxw, ly.inp, ly.outp});
+    const size_t nw = (size_t)ly.outp * ly.inp;
+    bool ok = dalloc((void**)&ly.W, 4 * nw) && dalloc((void**)&ly.mW, 4 * nw) && dalloc((void**)&ly.vW, 4 * nw) &&
+              dalloc((void**)&ly.G, 4 * nw) && dalloc((void**)&ly.b, 4 * ly.outp) &&
+              dalloc((void**)&ly.mb, 4 * ly.outp) && dalloc((void**)&ly.vb, 4 * ly.outp) &&
+              dalloc((void**)&ly.gb, 4 * ly.outp) && dalloc((void**)&ly.Wf, 2 * nw) && dalloc((void**)&ly.Wb, 2 * nw);
+    if (!ok) return fail(DGV_ERR_NOMEM, "hipMalloc(surrogate weights) failed");
+    // W ~ N(0, 1/fan_in), the last layer scaled by 0.1; padded entries zero
+    std::vector<float> W(nw, 0.f);
+    const float sd = (l + 1 == nlayers ? 0.1f : 1.0f) / std::sqrt((float)ly.in);
+    for (int o = 0; o < ly.out; ++o)
+      for (int i = 0; i < ly.in; ++i) W[(size_t)o * ly.inp + i] = sd * nd(rng);
+    if (hipMemcpy(ly.W, W.data(), 4 * nw, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(ly.mW, 0, 4 * nw) || hipMemset(ly.vW, 0, 4 * nw) || hipMemset(ly.G, 0, 4 * nw) ||
+        hipMemset(ly.b, 0, 4 * ly.outp) || hipMemset(ly.mb, 0, 4 * ly.outp) || hipMemset(ly.vb, 0, 4 * ly.outp) ||
+        hipMemset(ly.gb, 0, 4 * ly.outp))
+      return fail(DGV_ERR_HIP, "surrogate initialisation failed");
+    hipLaunchKernelGGL(k_weights_bf16, dim3(ly.inp / 64, ly.outp / 64), dim3(256), 0, ctx->stream, ly.outp, ly.inp,
+                       ly.W, ly.Wf, ly.Wb);
+    const Gemm f = plan_gemm(s->Bp, ly.outp, ly.inp, true);
+    const Gemm d = plan_gemm(s->Bp, ly.inp, ly.outp, true);
+    s->part_elems = std::max({s->part_elems, (int64_t)f.S * f.M * f.N, (int64_t)d.S * d.M * d.N});
+  }
+  s->H.assign(nlayers, nullptr);
+  s->Ht.assign(nlayers, nullptr);
+  s->Z.assign(std::max(nlayers - 1, 1), nullptr);
+  for (int l = 0; l < nlayers; ++l) {
+    const size_t n = (size_t)s->Bp * s->layers[l].inp;
+    if (!dalloc((void**)&s->H[l], 2 * n) || !dalloc((void**)&s->Ht[l], 2 * n))
+      return fail(DGV_ERR_NOMEM, "hipMalloc(activations) failed");
+    if (l + 1 < nlayers && !dalloc((void**)&s->Z[l], 4 * (size_t)s->Bp * s->layers[l].outp))
+      return fail(DGV_ERR_NOMEM, "hipMalloc(activations) failed");
+  }
+  const size_t nout = (size_t)s->Bp * s->layers[nlayers - 1].outp;
+  const size_t nmax = (size_t)s->Bp * maxw;
+  bool ok = dalloc((void**)&s->Y, 4 * nout) && dalloc((void**)&s->T, 4 * nout) && dalloc((void**)&s->dZ[0], 2 * nmax) &&
+            dalloc((void**)&s->dZ[1], 2 * nmax) && dalloc((void**)&s->dZt[0], 2 * nmax) &&
+            dalloc((void**)&s->dZt[1], 2 * nmax) && dalloc((void**)&s->part, 4 * (size_t)s->part_elems) &&
+            dalloc((void**)&s->dbpart, 4 * (size_t)(s->Bp / 64) * maxw) &&
+            dalloc((void**)&s->losspart, 8 * (size_t)(s->Bp / 64) * (maxw / 64)) && dalloc((void**)&s->d_loss, 8) &&
+            dalloc((void**)&s->d_res, 8 * (size_t)s->Bp);
+  if (!ok) return fail(DGV_ERR_NOMEM, "hipMalloc(surrogate buffers) failed");
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return fail(DGV_ERR_HIP, "surrogate initialisation kernels failed");
+  *out = s;
+  return DGV_OK;
+}
+
+void dgv_sur_destroy(dgv_surrogate* s) {
+  if (!s) return;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  free_surrogate(s);
+  delete s;
+}
+
+int dgv_sur_set_scales(dgv_surrogate* s, double f_scale, double q_scale) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (!(f_scale > 0) || !(q_scale > 0)) return serr(s, DGV_ERR_ARG, "scales must be positive");
+  s->f_scale = f_scale;
+  s->q_scale = q_scale;
+  return DGV_OK;
+}
+
+int dgv_sur_layer_shape(const dgv_surrogate* s, int32_t layer, int32_t* in, int32_t* out) {
+  if (!s || layer < 0 || layer >= s->L) { set_global_error("bad surrogate or layer"); return DGV_ERR_ARG; }
+  if (in) *in = s->layers[layer].in;
+  if (out) *out = s->layers[layer].out;
+  return DGV_OK;
+}
+
+static int layer_copy(dgv_surrogate* s, int32_t layer, float* W, float* b, const float* Wd, const float* bd,
+                      bool to_device) {
+  if (!s || layer < 0 || layer >= s->L) { set_global_error("bad surrogate or layer"); return DGV_ERR_ARG; }
+  SurLayer& ly = s->layers[layer];
+  SHIP(s, hipSetDevice(s->ctx->device));
+  SHIP(s, hipStreamSynchronize(s->ctx->stream));
+  const size_t rowb = sizeof(float) * ly.in, pitch = sizeof(float) * ly.inp;
+  if (to_device) {
+    if (W) SHIP(s, hipMemcpy2D(const_cast<float*>(Wd), pitch, W, rowb, rowb, ly.out, hipMemcpyHostToDevice));
+    if (b) SHIP(s, hipMemcpy(const_cast<float*>(bd), b, sizeof(float) * ly.out, hipMemcpyHostToDevice));
+  } else {
+    if (W) SHIP(s, hipMemcpy2D(W, rowb, Wd, pitch, rowb, ly.out, hipMemcpyDeviceToHost));
+    if (b) SHIP(s, hipMemcpy(b, bd, sizeof(float) * ly.out, hipMemcpyDeviceToHost));
+  }
+  return DGV_OK;
+}
+
+int dgv_sur_get_layer(dgv_surrogate* s, int32_t layer, float* W, float* b) {
+  return s ? layer_copy(s, layer, W, b, layer >= 0 && layer < s->L ? s->layers[layer].W : nullptr,
+                        layer >= 0 && layer < s->L ? s->layers[layer].b : nullptr, false)
+           : layer_copy(s, layer, W, b, nullptr, nullptr, false);
+}
+
+int dgv_sur_set_layer(dgv_surrogate* s, int32_t layer, const float* W, const float* b) {
+  if (!s || layer < 0 || layer >= s->L) { set_global_error("bad surrogate or layer"); return DGV_ERR_ARG; }
+  SurLayer& ly = s->layers[layer];
+  int rc = layer_copy(s, layer, const_cast<float*>(W), const_cast<float*>(b), ly.W, ly.b, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_weights_bf16, dim3(ly.inp / 64, ly.outp / 64), dim3(256), 0, s->ctx->stream, ly.outp, ly.inp,
+                     ly.W, ly.Wf, ly.Wb);
+  SHIP(s, hipGetLastError());
+  SHIP(s, hipStreamSynchronize(s->ctx->stream));
+  return DGV_OK;
+}
+
+int dgv_sur_get_grad(dgv_surrogate* s, int32_t layer, float* dW, float* db) {
+  if (!s || layer < 0 || layer >= s->L) { set_global_error("bad surrogate or layer"); return DGV_ERR_ARG; }
+  return layer_copy(s, layer, dW, db, s->layers[layer].G, s->layers[layer].gb, false);
+}
+
+int dgv_sur_train_step(dgv_surrogate* s, const double* d_f, const double* d_q, int64_t ld, int32_t nrows, double lr,
+                       double* loss) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (!d_f || !d_q || nrows < 1 || nrows > s->batch || ld < s->ctx->g.N || !(lr >= 0))
+    return serr(s, DGV_ERR_ARG, "train_step: bad argument");
+  SHIP(s, hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  int rc = load_rows(s, d_f, ld, nrows);
+  if (rc) return rc;
+  const SurLayer& lo = s->layers[s->L - 1];
+  const int64_t nt = (int64_t)s->Bp * lo.outp;
+  hipLaunchKernelGGL(k_rows_target, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, d_q, ld, nrows, lo.out,
+                     1.0 / s->q_scale, lo.outp, nt, s->T);
+  if ((rc = forward(s))) return rc;
+  const double inv_count = 1.0 / ((double)nrows * lo.out);
+  const int nlp = (lo.outp / 64) * (s->Bp / 64);
+  int cur = 0;
+  hipLaunchKernelGGL(k_loss, dim3(lo.outp / 64, s->Bp / 64), dim3(256), 0, st, s->Y, s->T, s->Bp, lo.outp, nrows,
+                     lo.out, (float)(2.0 * inv_count), s->dZ[cur], s->dZt[cur], s->dbpart, s->losspart);
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, st, s->losspart, nlp, inv_count, s->d_loss);
+  ++s->step;
+  AdamArgs h;
+  h.lr = (float)lr;
+  h.b1 = 0.9f;
+  h.b2 = 0.999f;
+  h.eps = 1e-8f;
+  h.c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->step)));
+  h.c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->step)));
+  for (int l = s->L - 1; l >= 0; --l) {
+    SurLayer& ly = s->layers[l];
+    hipLaunchKernelGGL(k_colsum, dim3((ly.outp + 255) / 256), dim3(256), 0, st, s->dbpart, s->Bp / 64, ly.outp, ly.gb);
+    if (l > 0) {  // gradient of the layer's input, through GELU of the previous layer
+      const Gemm g = plan_gemm(s->Bp, ly.inp, ly.outp, true);
+      if ((rc = launch_gemm(s, g, s->dZ[cur], ly.outp, ly.Wb, ly.outp, s->part))) return rc;
+      EpiArgs a{};
+      a.part = s->part;
+      a.S = g.S;
+      a.M = s->Bp;
+      a.N = ly.inp;
+      a.zin = s->Z[l - 1];
+      a.h = s->dZ[cur ^ 1];
+      a.ht = s->dZt[cur ^ 1];
+      a.dbpart = s->dbpart;
+      hipLaunchKernelGGL(k_epilogue, dim3(ly.inp / 64, s->Bp / 64), dim3(256), 0, st, a, (int)EPI_BWD_GELU);
+    }
+    const Gemm gw = plan_gemm(ly.outp, ly.inp, s->Bp, false);
+    if ((rc = launch_gemm(s, gw, s->dZt[cur], s->Bp, s->Ht[l], s->Bp, ly.G))) return rc;
+    if (lr > 0) {
+      hipLaunchKernelGGL(k_adam_w, dim3(ly.inp / 64, ly.outp / 64), dim3(256), 0, st, ly.outp, ly.inp, ly.G, ly.W,
+                         ly.mW, ly.vW, ly.Wf, ly.Wb, h);
+      hipLaunchKernelGGL(k_adam_vec, dim3((ly.outp + 255) / 256), dim3(256), 0, st, ly.outp, ly.gb, ly.b, ly.mb,
+                         ly.vb, h);
+    }
+    cur ^= 1;
+  }
+  SHIP(s, hipGetLastError());
+  if (loss) {
+    SHIP(s, hipMemcpyAsync(loss, s->d_loss, sizeof(double), hipMemcpyDeviceToHost, st));
+    SHIP(s, hipStreamSynchronize(st));
+  }
+  return DGV_OK;
+}
+
+int dgv_sur_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q, int32_t project) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (!d_f || !d_q || nrows < 0 || ld < s->ctx->g.N) return serr(s, DGV_ERR_ARG, "predict: bad argument");
+  SHIP(s, hipSetDevice(s->ctx->device));
+  for (int64_t r = 0; r < nrows; r += s->batch) {
+    const int nb = (int)std::min<int64_t>(s->batch, nrows - r);
+    int rc = predict_chunk(s, d_f + r * ld, ld, nb, d_q + r * ld, project);
+    if (rc) return rc;
+  }
+  return DGV_OK;
+}
+
+int dgv_sur_set_probe_cells(dgv_surrogate* s, const int32_t* cells, int32_t ncells) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (ncells < 1 || !cells) return serr(s, DGV_ERR_ARG, "at least one probe cell is needed");
+  SHIP(s, hipSetDevice(s->ctx->device));
+  const Grid& g = s->ctx->g;
+  std::vector<int32_t> cl(cells, cells + ncells);
+  std::sort(cl.begin(), cl.end());
+  cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+  for (int32_t c : cl)
+    if (c < 0 || c >= g.ncells) return serr(s, DGV_ERR_INDEX, "probe cell outside the grid");
+  if (!s->judge) {
+    std::vector<int32_t> goi(s->batch, 0);
+    dgv_ctx* grids[1] = {s->ctx};
+    int rc = dgv_batch_create(grids, 1, goi.data(), s->batch, DGV_FP64, &s->judge);
+    if (rc) return serr(s, rc, std::string("judge batch: ") + dgv_last_error(nullptr));
+  }
+  int rc = dgv_batch_set_probe_cells(s->judge, cl.data(), (int32_t)cl.size());
+  if (rc) return serr(s, rc, std::string("judge probe plan: ") + dgv_last_error(nullptr));
+  std::vector<int32_t> nodes;
+  for (int32_t c : cl)
+    for (int l = 0; l < g.D; ++l) nodes.push_back(c * g.D + l);
+  hipFree(s->d_probe_nodes);
+  s->d_probe_nodes = nullptr;
+  SHIP(s, hipMalloc(&s->d_probe_nodes, sizeof(int32_t) * nodes.size()));
+  SHIP(s, hipMemcpy(s->d_probe_nodes, nodes.data(), sizeof(int32_t) * nodes.size(), hipMemcpyHostToDevice));
+  s->nprobe_nodes = (int)nodes.size();
+  s->probe_cells = cl;
+  return DGV_OK;
+}
+
+int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q, double tol,
+                 double* residual, int32_t* fallback, int64_t* nfallback) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (!d_f || !d_q || nrows < 0 || ld < s->ctx->g.N || !(tol >= 0)) return serr(s, DGV_ERR_ARG, "eval: bad argument");
+  if (!s->judge || s->nprobe_nodes == 0) return serr(s, DGV_ERR_STATE, "no probe cells set");
+  SHIP(s, hipSetDevice(s->ctx->device));
+  dgv_ctx* ctx = s->ctx;
+  double *jf = nullptr, *jq = nullptr;
+  int64_t off = 0, stride = 0;
+  int rc = dgv_batch_device_buffers(s->judge, &jf, &jq);
+  if (!rc) rc = dgv_batch_layout(s->judge, 0, &off, &stride, nullptr);
+  if (rc) return serr(s, rc, dgv_last_error(nullptr));
+  std::vector<double> res(s->batch);
+  int64_t nfb = 0;
+  for (int64_t r = 0; r < nrows; r += s->batch) {
+    const int nb = (int)std::min<int64_t>(s->batch, nrows - r);
+    const double* F = d_f + r * ld;
+    double* Q = d_q + r * ld;
+    if ((rc = predict_chunk(s, F, ld, nb, Q, 1))) return rc;
+    hipLaunchKernelGGL(k_rows_to_packed, dim3((unsigned)((ctx->g.N + 63) / 64), (unsigned)((nb + 63) / 64)), dim3(256),
+                       0, ctx->stream, F, ld, nb, (int)ctx->g.N, jf + off, stride);
+    SHIP(s, hipGetLastError());
+    SHIP(s, hipStreamSynchronize(ctx->stream));
+    if ((rc = dgv_batch_eval_q_probe(s->judge))) return serr(s, rc, dgv_last_error(nullptr));
+    hipLaunchKernelGGL(k_probe_residual, dim3(nb), dim3(256), 0, ctx->stream, Q, ld, jq + off, stride,
+                       s->d_probe_nodes, s->nprobe_nodes, s->d_res);
+    SHIP(s, hipGetLastError());
+    SHIP(s, hipMemcpyAsync(res.data(), s->d_res, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
+    SHIP(s, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nb; ++i) {
+      const bool fb = !(res[i] <= tol);
+      if (residual) residual[r + i] = res[i];
+      if (fallback) fallback[r + i] = fb ? 1 : 0;
+      if (fb) {
+        ++nfb;
+        if ((rc = eval_q_dev(ctx, F + (int64_t)i * ld, Q + (int64_t)i * ld, 1.0))) return rc;
+      }
+    }
+  }
+  SHIP(s, hipStreamSynchronize(ctx->stream));
+  if (nfallback) *nfallback = nfb;
+  return DGV_OK;
+}
+
+int dgv_sur_time_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q, int32_t niter,
+                         double* ms_per_call) {
+  if (!s || niter < 1) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  dgv_ctx* ctx = s->ctx;
+  SHIP(s, hipSetDevice(ctx->device));
+  SHIP(s, hipEventRecord(ctx->ev[0], ctx->stream));
+  for (int i = 0; i < niter; ++i) {
+    int rc = dgv_sur_predict(s, d_f, ld, nrows, d_q, 1);
+    if (rc) return rc;
+  }
+  SHIP(s, hipEventRecord(ctx->ev[1], ctx->stream));
+  SHIP(s, hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  SHIP(s, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  if (ms_per_call) *ms_per_call = ms / niter;
+  return DGV_OK;
+}
+
+}  // extern "C"

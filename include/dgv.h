@@ -184,6 +184,50 @@ int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_
 int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms);
 /* device pointers of the packed f and q buffers */
 int dgv_batch_device_buffers(dgv_batch* b, double** d_f, double** d_q);
+/* Q of every instance on a set of probe cells only (0-based cell ids, valid on
+ * every grid): items of the other cells are not planned, so the cost is the
+ * probe's share of a full evaluation.  q at the probe cells' nodes is written
+ * into the packed q buffer, other nodes are left as they were. */
+int dgv_batch_set_probe_cells(dgv_batch* b, const int32_t* cells, int32_t ncells);
+int dgv_batch_eval_q_probe(dgv_batch* b);
+
+/* ---- learned surrogate ------------------------------------------------------------ */
+/* No counterpart in the reference.  An MLP widths[0] -> ... -> widths[nlayers]
+ * with GELU between layers (widths[0] = widths[nlayers] = nodes of the context's
+ * grid) maps f / f_scale to Q(f) / q_scale.  BF16 weights and activations on the
+ * matrix cores with FP32 accumulation, FP32 master weights, Adam (0.9, 0.999,
+ * 1e-8).  The output is scaled back to FP64 and projected onto the complement
+ * of the collision invariants, so that sum_i w_i psi_k q_i = 0 to rounding.
+ * Rows are VDFs in reference node order, row-major with leading dimension ld
+ * (doubles); every row pointer is a device pointer. */
+typedef struct dgv_surrogate dgv_surrogate;
+int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t batch, uint64_t seed,
+                   dgv_surrogate** out);
+void dgv_sur_destroy(dgv_surrogate* s);
+int dgv_sur_set_scales(dgv_surrogate* s, double f_scale, double q_scale);
+int dgv_sur_layer_shape(const dgv_surrogate* s, int32_t layer, int32_t* in, int32_t* out);
+/* FP32 master weights W[out][in] and bias[out] of one layer (host arrays) */
+int dgv_sur_get_layer(dgv_surrogate* s, int32_t layer, float* W, float* b);
+int dgv_sur_set_layer(dgv_surrogate* s, int32_t layer, const float* W, const float* b);
+/* gradients of the last training step's loss */
+int dgv_sur_get_grad(dgv_surrogate* s, int32_t layer, float* dW, float* db);
+/* one forward and backward pass over nrows <= batch rows of (f, Q); the loss is
+ * the mean squared error of the scaled output over the nrows x N entries; lr > 0
+ * applies one Adam step, lr = 0 leaves the weights unchanged */
+int dgv_sur_train_step(dgv_surrogate* s, const double* d_f, const double* d_q, int64_t ld, int32_t nrows,
+                       double lr, double* loss);
+/* surrogate Q of nrows rows (batch rows per pass); project = 1 applies the FP64 projection */
+int dgv_sur_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q,
+                    int32_t project);
+/* The direct evaluator as judge and fallback: on the probe cells the batched
+ * FP64 engine gives each row's residual ||q_sur - q_direct|| / ||q_direct||;
+ * rows whose residual exceeds tol receive the direct FP64 Q on every node.
+ * residual and fallback (host, nrows entries) may be NULL. */
+int dgv_sur_set_probe_cells(dgv_surrogate* s, const int32_t* cells, int32_t ncells);
+int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q, double tol,
+                 double* residual, int32_t* fallback, int64_t* nfallback);
+int dgv_sur_time_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q,
+                         int32_t niter, double* ms_per_call);
 
 /* ---- stream / timing helpers ----------------------------------------------------- */
 int dgv_ctx_synchronize(dgv_ctx* ctx);
