@@ -93,6 +93,7 @@ SIGNATURES = {
     "dgv_batch_rk_run": (C.c_int, [_vp, _dbl, _dbl, _i32, _i64, C.c_int]),
     "dgv_batch_autotune": (C.c_int, [_vp, _ip, _dp]),
     "dgv_batch_device_buffers": (C.c_int, [_vp, _pp, _pp]),
+    "dgv_batch_project_invariants": (C.c_int, [_vp]),
     "dgv_batch_set_probe_cells": (C.c_int, [_vp, _ip, _i32]),
     "dgv_batch_eval_q_probe": (C.c_int, [_vp]),
     "dgv_sur_create": (C.c_int, [_vp, _ip, _i32, _i32, C.c_uint64, _pp]),

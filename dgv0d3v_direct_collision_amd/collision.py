@@ -398,6 +398,11 @@ class DGVBatch:
     def rk_run(self, dt: float, nsteps: int, rkmts: int = 2, coef: float = 1.0, use_graph=True):
         self._chk(self._L.dgv_batch_rk_run(self._b, dt, coef, rkmts, nsteps, int(use_graph)))
 
+    def project_invariants(self):
+        """FP64 projection of every instance's Q onto the complement of the
+        collision invariants (in place on the device)."""
+        self._chk(self._L.dgv_batch_project_invariants(self._b))
+
     def set_probe_cells(self, cells):
         """Cells (0-based) evaluated by ``eval_q_probe``."""
         c = np.ascontiguousarray(cells, np.int32)
@@ -413,7 +418,18 @@ class DGVBatch:
         self._chk(self._L.dgv_batch_autotune(self._b, C.byref(best), C.byref(ms)))
         return best.value, ms.value
 
-    def time_eval(self, niter: int = 3) -> float:
+    def time_eval(self, niter: int = 3, main_kernel: bool = False):
+        """Milliseconds per batched evaluation (HIP events on the batch's
+        stream); with main_kernel, also those of the persistent kernel alone."""
         a = C.c_double()
-        self._chk(self._L.dgv_batch_time_eval(self._b, niter, C.byref(a), None))
-        return a.value
+        m = C.c_double()
+        self._chk(self._L.dgv_batch_time_eval(self._b, niter, C.byref(a), C.byref(m) if main_kernel else None))
+        return (a.value, m.value) if main_kernel else a.value
+
+    def device_buffers(self):
+        """(device pointer of packed f, of packed q, offset of instance 0, stride)."""
+        df, dq = C.c_void_p(), C.c_void_p()
+        self._chk(self._L.dgv_batch_device_buffers(self._b, C.byref(df), C.byref(dq)))
+        off, stride, total = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(self._L.dgv_batch_layout(self._b, 0, C.byref(off), C.byref(stride), C.byref(total)))
+        return df.value, dq.value, off.value, stride.value

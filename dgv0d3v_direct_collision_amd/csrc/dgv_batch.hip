@@ -62,6 +62,14 @@ struct BatchPlan {
   int32_t* d_red_key = nullptr;    // (grid, cell, chunk) per reduction task
 };
 
+// per grid: what the batched projection needs
+struct BatchProjGrid {
+  const double *u, *v, *w, *gw;
+  double L[25];                    // Cholesky factor of the invariants' Gram matrix
+  int64_t foff, N;
+  int32_t Bpad, nin;
+};
+
 }  // namespace dgv
 
 struct dgv_batch {
@@ -84,6 +92,9 @@ struct dgv_batch {
   dgv::BatchPlan plan;             // all cells
   dgv::BatchPlan probe;            // the probe cells (empty until set)
   std::vector<int32_t> probe_cells;
+  dgv::BatchProjGrid* d_proj = nullptr;   // projection tables per grid
+  int32_t* d_proj_task = nullptr;         // (grid, chunk) per block
+  int32_t nproj_task = 0;
   int* d_counter = nullptr;
   int ncu = 256;
   hipStream_t stream = nullptr;
@@ -316,6 +327,61 @@ __global__ void k_blincomb(int64_t n, double* __restrict__ out, BLinComb c) {
   out[i] = s;
 }
 
+// FP64 projection of every instance's Q onto the complement of the collision
+// invariants psi = 1,u,v,w,|u|^2 in the w-weighted inner product.  One block of
+// 16 waves per (grid, chunk of 64 instances): lane = instance, wave k sums the
+// moments over its share of the nodes, the shares are added in wave order, and
+// each lane solves G lam = m with the grid's Cholesky factor and applies it.
+__global__ __launch_bounds__(1024) void k_batch_project(const BatchProjGrid* __restrict__ grids,
+                                                        const int32_t* __restrict__ task, double* __restrict__ q) {
+  __shared__ double part[16][5][64];
+  const int g = task[2 * blockIdx.x], vc = task[2 * blockIdx.x + 1];
+  const BatchProjGrid G = grids[g];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int inst = vc * 64 + lane;
+  double* Q = q + G.foff + inst;
+  const int64_t per = (G.N + 15) / 16;
+  const int64_t i0 = wave * per, i1 = min(G.N, i0 + per);
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int64_t i = i0; i < i1; ++i) {
+    const double x = G.gw[i] * Q[i * G.Bpad];
+    const double uu = G.u[i], vv = G.v[i], ww = G.w[i];
+    s[0] += x;
+    s[1] += x * uu;
+    s[2] += x * vv;
+    s[3] += x * ww;
+    s[4] += x * (uu * uu + vv * vv + ww * ww);
+  }
+  for (int k = 0; k < 5; ++k) part[wave][k][lane] = s[k];
+  __syncthreads();
+  double lam[5];
+  {
+    double m[5];
+    for (int k = 0; k < 5; ++k) {
+      double a = 0.0;
+      for (int wv = 0; wv < 16; ++wv) a += part[wv][k][lane];
+      m[k] = a;
+    }
+    double y[5];
+    for (int i = 0; i < 5; ++i) {
+      double a = m[i];
+      for (int k = 0; k < i; ++k) a -= G.L[i * 5 + k] * y[k];
+      y[i] = a / G.L[i * 5 + i];
+    }
+    for (int i = 4; i >= 0; --i) {
+      double a = y[i];
+      for (int k = i + 1; k < 5; ++k) a -= G.L[k * 5 + i] * lam[k];
+      lam[i] = a / G.L[i * 5 + i];
+    }
+  }
+  if (inst >= G.nin) return;
+  for (int64_t i = i0; i < i1; ++i) {
+    const double uu = G.u[i], vv = G.v[i], ww = G.w[i];
+    Q[i * G.Bpad] = Q[i * G.Bpad] - (lam[0] + lam[1] * uu + lam[2] * vv + lam[3] * ww +
+                                     lam[4] * (uu * uu + vv * vv + ww * ww));
+  }
+}
+
 static void destroy_plan(dgv_batch* b, BatchPlan& P) {
   hipFree(P.d_items); hipFree(P.d_partial); hipFree(P.d_red_rows); hipFree(P.d_red_ptr); hipFree(P.d_red_key);
   P.d_items = nullptr; P.d_partial = nullptr; P.d_red_rows = nullptr; P.d_red_ptr = nullptr; P.d_red_key = nullptr;
@@ -410,7 +476,8 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
   return DGV_OK;
 }
 
-static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, double* d_q, double scale) {
+static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, double* d_q, double scale,
+                          bool reduce = true) {
   const int D = b->grids[0]->g.D;
   const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
@@ -436,7 +503,7 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
                            b->d_f32, P.d_partial, b->d_counter);
     }
   }
-  if (P.nred > 0) {
+  if (reduce && P.nred > 0) {
     if (D == 27)
       hipLaunchKernelGGL(k_batch_reduce<27>, dim3((unsigned)P.nred), dim3(64), 0, st, P.nred, P.d_red_key,
                          P.d_red_ptr, P.d_red_rows, b->d_grids, P.d_partial, scale, d_q);
@@ -611,6 +678,26 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     gd[g].Mw = ctx->g.Mw;
     gd[g].D = D;
   }
+  {
+    std::vector<BatchProjGrid> pg(ngrids);
+    std::vector<int32_t> task;
+    for (int g = 0; g < ngrids; ++g) {
+      const dgv_ctx* ctx = grids[g];
+      pg[g].u = ctx->d_u; pg[g].v = ctx->d_v; pg[g].w = ctx->d_w; pg[g].gw = ctx->d_gwts;
+      if (gram_cholesky(ctx->g, pg[g].L)) return fail(DGV_ERR_GRID, "Gram matrix of the invariants is singular");
+      pg[g].foff = b->foff[g];
+      pg[g].N = ctx->g.N;
+      pg[g].Bpad = b->Bpad[g];
+      pg[g].nin = b->nin[g];
+      for (int vc = 0; vc < b->Bpad[g] / 64; ++vc) { task.push_back(g); task.push_back(vc); }
+    }
+    b->nproj_task = (int32_t)(task.size() / 2);
+    if (hipMalloc(&b->d_proj, sizeof(BatchProjGrid) * ngrids) ||
+        hipMemcpy(b->d_proj, pg.data(), sizeof(BatchProjGrid) * ngrids, hipMemcpyHostToDevice) ||
+        hipMalloc(&b->d_proj_task, sizeof(int32_t) * std::max<size_t>(task.size(), 2)) ||
+        hipMemcpy(b->d_proj_task, task.data(), sizeof(int32_t) * task.size(), hipMemcpyHostToDevice))
+      return fail(DGV_ERR_HIP, "projection tables");
+  }
   if (hipMalloc(&b->d_grids, sizeof(BatchGridDev) * ngrids) ||
       hipMemcpy(b->d_grids, gd.data(), sizeof(BatchGridDev) * ngrids, hipMemcpyHostToDevice))
     return fail(DGV_ERR_HIP, "grid table");
@@ -628,6 +715,7 @@ void dgv_batch_destroy(dgv_batch* b) {
   destroy_plan(b, b->plan);
   destroy_plan(b, b->probe);
   hipFree(b->d_f); hipFree(b->d_q); hipFree(b->d_f32); hipFree(b->d_counter); hipFree(b->d_grids);
+  hipFree(b->d_proj); hipFree(b->d_proj_task);
   for (int i = 0; i < 4; ++i) hipFree(b->d_tmp[i]);
   for (auto p : b->d_blist) hipFree(p);
   for (auto p : b->d_pairs32) hipFree(p);
@@ -676,6 +764,17 @@ int dgv_batch_eval_q(dgv_batch* b) {
   BHIP(b, hipSetDevice(b->grids[0]->device));
   int rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0);
   if (rc) return rc;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_project_invariants(dgv_batch* b) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  if (b->nproj_task > 0)
+    hipLaunchKernelGGL(k_batch_project, dim3((unsigned)b->nproj_task), dim3(1024), 0, b->stream, b->d_proj,
+                       b->d_proj_task, b->d_q);
+  BHIP(b, hipGetLastError());
   BHIP(b, hipStreamSynchronize(b->stream));
   return DGV_OK;
 }
@@ -748,11 +847,18 @@ int dgv_batch_time_eval(dgv_batch* b, int32_t niter, double* ms_per_call, double
     if (rc) return rc;
   }
   BHIP(b, hipEventRecord(b->ev[1], b->stream));
-  BHIP(b, hipEventSynchronize(b->ev[1]));
-  float a = 0;
+  // the persistent main kernel alone (with its FP32 conversion and queue reset)
+  for (int i = 0; i < niter && ms_main_kernel; ++i) {
+    int rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0, false);
+    if (rc) return rc;
+  }
+  BHIP(b, hipEventRecord(b->ev[2], b->stream));
+  BHIP(b, hipEventSynchronize(b->ev[2]));
+  float a = 0, m = 0;
   BHIP(b, hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+  BHIP(b, hipEventElapsedTime(&m, b->ev[1], b->ev[2]));
   if (ms_per_call) *ms_per_call = a / niter;
-  if (ms_main_kernel) *ms_main_kernel = a / niter;
+  if (ms_main_kernel) *ms_main_kernel = m / niter;
   return DGV_OK;
 }
 
@@ -760,18 +866,21 @@ int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
   if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
   BHIP(b, hipSetDevice(b->grids[0]->device));
   const int32_t tiles[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
-  const int32_t wpcs[] = {4, 8, 16};
+  const int32_t wpcs[] = {4, 8};
   int32_t bt = 0, bw = 0;
   double bms = 1e300;
+  bool warm = false;
   for (int32_t t : tiles) {
     int rc = build_batch_plan(b, b->plan, t);
     if (rc) return rc;
     for (int32_t w : wpcs) {
       b->waves_per_cu = w;
       double ms;
-      rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0);
-      if (rc) return rc;
-      rc = dgv_batch_time_eval(b, 2, &ms, nullptr);
+      if (!warm) {
+        if ((rc = batch_eval_dev(b, b->plan, b->d_f, b->d_q, 1.0))) return rc;
+        warm = true;
+      }
+      rc = dgv_batch_time_eval(b, 1, &ms, nullptr);
       if (rc) return rc;
       if (ms < bms) { bms = ms; bt = t; bw = w; }
     }

@@ -184,6 +184,9 @@ int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_
 int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms);
 /* device pointers of the packed f and q buffers */
 int dgv_batch_device_buffers(dgv_batch* b, double** d_f, double** d_q);
+/* FP64 projection of every instance's q (packed buffer) onto the complement of
+ * the collision invariants, as dgv_project_invariants does for one VDF */
+int dgv_batch_project_invariants(dgv_batch* b);
 /* Q of every instance on a set of probe cells only (0-based cell ids, valid on
  * every grid): items of the other cells are not planned, so the cost is the
  * probe's share of a full evaluation.  q at the probe cells' nodes is written

@@ -72,3 +72,42 @@ def test_batch_rk_graph_matches_single(grids):
             grids[g].TimeIntegratorORK_SH_DGV(ref, 0.02)
         assert rel_l2(b.get_f(i), ref) <= 1e-12
     b.close()
+
+
+def test_batch_probe_cells_and_projection(grids):
+    import dgv0d3v_direct_collision_amd as dgv
+    rng = np.random.default_rng(9)
+    which = np.array([1] * 70 + [0] * 5, np.int32)     # two 64-instance chunks on grid 1
+    b = dgv.DGVBatch(grids, which, dgv.DGV_FP32_COMP)
+    fs = [vdf(grids[g], rng, 1 + i % 4) for i, g in enumerate(which)]
+    for i, f in enumerate(fs):
+        b.set_f(i, f)
+    b.eval_q()
+    full = [b.get_q(i) for i in range(len(which))]
+    # probe cells: only their nodes are rewritten, with the same values
+    cells = [0, 7, 13, 26, 7]                          # valid on both grids
+    for i in range(len(which)):
+        b.set_f(i, 2.0 * fs[i])                          # Q(2f) = 4 Q(f)
+    b.set_probe_cells(cells)
+    b.eval_q_probe()
+    for i, g in enumerate(which):
+        D = grids[g].D
+        nodes = np.concatenate([np.arange(c * D, c * D + D) for c in sorted(set(cells))])
+        q = b.get_q(i)
+        other = np.setdiff1d(np.arange(grids[g].nnodes), nodes)
+        assert np.array_equal(q[other], full[i][other])
+        assert rel_l2(q[nodes], 4.0 * full[i][nodes]) <= 1e-6
+    # FP64 projection of every instance: moments vanish to rounding
+    b.eval_q()
+    raw = [b.get_q(i) for i in range(len(which))]
+    b.project_invariants()
+    for i, g in enumerate(which):
+        G = grids[g]
+        q = b.get_q(i)
+        psi = np.stack([np.ones(G.nnodes), G.nodes_u, G.nodes_v, G.nodes_w,
+                        G.nodes_u ** 2 + G.nodes_v ** 2 + G.nodes_w ** 2])
+        mom = (q * G.nodes_gwts) @ psi.T
+        scale = np.abs(q * G.nodes_gwts) @ np.abs(psi.T)
+        assert np.all(np.abs(mom) <= 1e-13 * scale)
+        assert rel_l2(q, G.project_invariants(raw[i])) <= 1e-13
+    b.close()
