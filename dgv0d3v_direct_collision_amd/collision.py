@@ -208,6 +208,15 @@ class DGV:
     def read_aarray(self, path: str):
         self._chk(self._L.dgv_ctx_read_aarray_file(self._ctx, path.encode()))
 
+    def TruncAarrsTrhls_DGV(self, trhld: float):
+        """Drops the records with |A| <= trhld (dgvtools_mod.f90:1313-1362)."""
+        self._chk(self._L.dgv_ctx_trunc_aarray_threshold(self._ctx, trhld))
+
+    def TruncAarrsRadius_DGV(self, rad: float):
+        """Drops the records whose nodes lie more than rad apart
+        (dgvtools_mod.f90:1375-1429)."""
+        self._chk(self._L.dgv_ctx_trunc_aarray_radius(self._ctx, rad))
+
     def set_kernel(self, kernel: int):
         self._chk(self._L.dgv_ctx_set_kernel(self._ctx, kernel))
 

@@ -113,6 +113,13 @@ int dgv_ctx_get_aarray_basis(const dgv_ctx* ctx, int32_t l, double* A, int32_t* 
  * (readwrite.f90:1066-1118,1459-1484). */
 int dgv_aarray_write_file(const dgv_ctx* ctx, const char* path);
 int dgv_ctx_read_aarray_file(dgv_ctx* ctx, const char* path);
+/* TruncAarrsTrhls_DGV(trhld) (dgvtools_mod.f90:1313-1362): keeps the records with
+ * |A| > trhld.  TruncAarrsRadius_DGV(rad) (dgvtools_mod.f90:1375-1429): keeps
+ * the records whose nodes xi, xi1 lie at most rad apart.  Both work on the
+ * device copy in place and keep the records' order; batches and surrogates
+ * built on the context before the call must be created again. */
+int dgv_ctx_trunc_aarray_threshold(dgv_ctx* ctx, double trhld);
+int dgv_ctx_trunc_aarray_radius(dgv_ctx* ctx, double rad);
 /* reference chunk-file base name (MakeBaseNameAoperChnksDGV, readwrite.f90:2224-2281) */
 int dgv_aarray_chunk_name(const dgv_grid_desc* grid, const char* dir, const char* base,
                           int32_t chunk, int32_t k_c, int32_t N_x, char* out, size_t cap);

@@ -21,6 +21,8 @@ function               reference routine restated
 ``rk_step``            SpatialOperSH_DGV run_mode 0 + TimeIntegratorORK_SH_DGV
                        (spat_oper_mod.f90:28-68, time_integr_mod.f90:26-152)
 ``ab_step``            TimeIntegratorMTS_SH_DGV (time_integr_mod.f90:314-364)
+``trunc_threshold``     TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362)
+``trunc_radius``        TruncAarrsRadius_DGV (dgvtools_mod.f90:1375-1429)
 ``seta`` / ``seta_raw``  SetA_DGV with the Lagrange basis restored, its adaptive
                        Simpson integration in chi and epsilon (dgvtools_mod.f90:377-681,
                        938-1122, 232-289)
@@ -122,6 +124,10 @@ def lib():
         L.orc_seta.argtypes = [C.POINTER(_Geom), C.c_int, C.c_int32, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_int64, _f64p, _i32p, _i32p, C.c_int]
         L.orc_seta.restype = C.c_int64
+        L.orc_trunc_threshold.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p, C.c_double]
+        L.orc_trunc_threshold.restype = C.c_int64
+        L.orc_trunc_radius.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, C.c_double]
+        L.orc_trunc_radius.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -317,3 +323,19 @@ def seta(G: Geometry, i1, Trad=100.0, ErrChi=1e-6, ErrEps=1e-6, min_sens=0.0, mo
     if n < 0:
         raise RuntimeError(f"orc_seta failed ({n})")
     return A[:n].copy(), xi[:n].copy(), xi1[:n].copy()
+
+
+def trunc_threshold(capphi, A, A_xi, A_xi1, trhld):
+    """TruncAarrsTrhls_DGV: returns new (capphi, A, A_xi, A_xi1)."""
+    cap = np.array(capphi, np.int32)
+    a, x, y = np.array(A, np.float64), np.array(A_xi, np.int32), np.array(A_xi1, np.int32)
+    n = lib().orc_trunc_threshold(len(cap), cap, a, x, y, trhld)
+    return cap, a[:n].copy(), x[:n].copy(), y[:n].copy()
+
+
+def trunc_radius(G: "Geometry", capphi, A, A_xi, A_xi1, rad):
+    """TruncAarrsRadius_DGV: returns new (capphi, A, A_xi, A_xi1)."""
+    cap = np.array(capphi, np.int32)
+    a, x, y = np.array(A, np.float64), np.array(A_xi, np.int32), np.array(A_xi1, np.int32)
+    n = lib().orc_trunc_radius(G.ptr, cap, a, x, y, rad)
+    return cap, a[:n].copy(), x[:n].copy(), y[:n].copy()

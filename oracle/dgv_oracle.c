@@ -840,3 +840,51 @@ int64_t orc_seta(const orc_geom* G, int mode, int32_t i1, double Trad, double Er
   if (err) return -(int64_t)err;
   return tot;
 }
+
+/* ---- A-array truncation ------------------------------------------------------
+ * Both routines walk the basis functions in order and copy the surviving
+ * records forward (the reference's compaction through scratch arrays,
+ * dgvtools_mod.f90:1339-1359 and 1405-1426). */
+int64_t orc_trunc_threshold(int64_t ncap, int32_t* capphi, double* A, int32_t* A_xi,
+                            int32_t* A_xi1, double trhld) {
+  int64_t src = 0, dst = 0;
+  for (int64_t i = 0; i < ncap; ++i) {
+    int32_t kept = 0;
+    for (int32_t j = 0; j < capphi[i]; ++j, ++src) {
+      if (fabs(A[src]) > trhld) {
+        A[dst] = A[src];
+        A_xi[dst] = A_xi[src];
+        A_xi1[dst] = A_xi1[src];
+        ++dst;
+        ++kept;
+      }
+    }
+    capphi[i] = kept;
+  }
+  return dst;
+}
+
+int64_t orc_trunc_radius(const orc_geom* G, int32_t* capphi, double* A, int32_t* A_xi,
+                         int32_t* A_xi1, double rad) {
+  const double radsq = rad * rad;
+  int64_t src = 0, dst = 0;
+  for (int64_t i = 0; i < G->nnodes; ++i) {
+    int32_t kept = 0;
+    for (int32_t j = 0; j < capphi[i]; ++j, ++src) {
+      const int64_t a = A_xi[src] - 1, b = A_xi1[src] - 1;
+      const double du = G->nodes_u[a] - G->nodes_u[b];
+      const double dv = G->nodes_v[a] - G->nodes_v[b];
+      const double dw = G->nodes_w[a] - G->nodes_w[b];
+      const double dsq = du * du + dv * dv + dw * dw;
+      if (dsq <= radsq) {
+        A[dst] = A[src];
+        A_xi[dst] = A_xi[src];
+        A_xi1[dst] = A_xi1[src];
+        ++dst;
+        ++kept;
+      }
+    }
+    capphi[i] = kept;
+  }
+  return dst;
+}

@@ -110,6 +110,15 @@ int64_t orc_seta(const orc_geom* G, int mode, int32_t i1, double Trad,
                  double ErrChi, double ErrEps, double min_sens, int64_t cap,
                  double* A, int32_t* A_xi, int32_t* A_xi1, int nthreads);
 
+/* TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362): keep records with
+ * |A| > trhld, in order; capphi updated in place.  Returns the new count.     */
+int64_t orc_trunc_threshold(int64_t ncap, int32_t* capphi, double* A, int32_t* A_xi,
+                            int32_t* A_xi1, double trhld);
+/* TruncAarrsRadius_DGV (dgvtools_mod.f90:1375-1429): keep records whose nodes
+ * (1-based ids) are at squared distance <= rad**2.  Returns the new count.    */
+int64_t orc_trunc_radius(const orc_geom* G, int32_t* capphi, double* A, int32_t* A_xi,
+                         int32_t* A_xi1, double rad);
+
 #ifdef __cplusplus
 }
 #endif
