@@ -73,6 +73,8 @@ SIGNATURES = {
     "dgv_eval_q": (C.c_int, [_vp, _dp, _dp, _i64]),
     "dgv_eval_q_dev": (C.c_int, [_vp, _vp, _vp, _i64]),
     "dgv_eval_q_a": (C.c_int, [_vp, _dp, _dp, _i64]),
+    "dgv_eval_mixed": (C.c_int, [_vp, _dp, _dp, _dp, _i64]),
+    "dgv_eval_mixed_dev": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),
     "dgv_ctx_set_kernel": (C.c_int, [_vp, C.c_int]),
     "dgv_ctx_set_tile": (C.c_int, [_vp, _i32]),
     "dgv_ctx_autotune": (C.c_int, [_vp, _ip, _dp]),

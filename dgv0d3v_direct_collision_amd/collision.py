@@ -244,6 +244,14 @@ class DGV:
         self._chk(self._L.dgv_eval_q_a(self._ctx, dptr(f), dptr(fc), self.nnodes))
         return fc
 
+    def EvalCollisionPeriodicMixedTermsA_DGV(self, f, fm) -> np.ndarray:
+        """Cross term 2/w sum A (fm(xi) f(xi1) + f(xi) fm(xi1)) (collision_mod.f90:244-332)."""
+        f = self._f(f)
+        fm = self._f(fm)
+        fc = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_eval_mixed(self._ctx, dptr(f), dptr(fm), dptr(fc), self.nnodes))
+        return fc
+
     def SpatialOperSH_DGV(self, f, dt: float) -> np.ndarray:
         """spat_oper_mod.f90:28-68, run_mode 0"""
         f = self._f(f)

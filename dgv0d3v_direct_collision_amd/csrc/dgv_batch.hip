@@ -357,19 +357,24 @@ __global__ __launch_bounds__(1024) void k_batch_project(const BatchProjGrid* __r
   double lam[5];
   {
     double m[5];
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
       double a = 0.0;
       for (int wv = 0; wv < 16; ++wv) a += part[wv][k][lane];
       m[k] = a;
     }
     double y[5];
+#pragma unroll
     for (int i = 0; i < 5; ++i) {
       double a = m[i];
+#pragma unroll
       for (int k = 0; k < i; ++k) a -= G.L[i * 5 + k] * y[k];
       y[i] = a / G.L[i * 5 + i];
     }
+#pragma unroll
     for (int i = 4; i >= 0; --i) {
       double a = y[i];
+#pragma unroll
       for (int k = i + 1; k < 5; ++k) a -= G.L[k * 5 + i] * lam[k];
       lam[i] = a / G.L[i * 5 + i];
     }

@@ -332,6 +332,27 @@ int dgv_eval_q(dgv_ctx* ctx, const double* f, double* fc, int64_t n) {
 
 int dgv_eval_q_a(dgv_ctx* ctx, const double* f, double* fc, int64_t n) { return dgv_eval_q(ctx, f, fc, n); }
 
+int dgv_eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* d_fc, int64_t n) {
+  CHECK_CTX(ctx);
+  CHECK_N(ctx, n);
+  return eval_mixed_dev(ctx, d_f, d_fm, d_fc);
+}
+
+int dgv_eval_mixed(dgv_ctx* ctx, const double* f, const double* fm, double* fc, int64_t n) {
+  CHECK_CTX(ctx);
+  CHECK_N(ctx, n);
+  if (!f || !fm || !fc) return set_error(ctx, DGV_ERR_ARG, "null array");
+  const size_t nb = sizeof(double) * n;
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
+  DGV_HIP(ctx, hipMemcpyAsync(ctx->d_tmp[5], fm, nb, hipMemcpyHostToDevice, ctx->stream));
+  int rc = eval_mixed_dev(ctx, ctx->d_f, ctx->d_tmp[5], ctx->d_q);
+  if (rc) return rc;
+  DGV_HIP(ctx, hipMemcpyAsync(fc, ctx->d_q, nb, hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
 int dgv_ctx_autotune(dgv_ctx* ctx, int32_t* best_pairs_per_item, double* best_ms) {
   CHECK_CTX(ctx);
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set");

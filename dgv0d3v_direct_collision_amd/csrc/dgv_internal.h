@@ -164,6 +164,7 @@ int ensure_record_layout(dgv_ctx* ctx);
 int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double scale, bool mfma);
 int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
+int eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* d_q);
 int launch_axpy(dgv_ctx* ctx, int64_t n, double* out, const double* x0, int nterms,
                 const double* const* xs, const double* coefs);
 int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);

@@ -18,15 +18,20 @@ namespace dgv {
 //   acc[l] += T[l] * g.
 // Bounds tests of collision_mod.f90:196-211 are unnecessary: every cell of a
 // group's box is in range for every block of the group by construction.
-template <int D>
+// MIX: the cross term of EvalCollisionPeriodicMixedTermsA_DGV
+// (collision_mod.f90:244-332) with g = fm: the pair's product is
+// g[c+a][p] f[c+b][q] + f[c+a][p] g[c+b][q].
+template <int D, bool MIX>
 __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ items,
                                                 const Group* __restrict__ groups,
                                                 const int32_t* __restrict__ gblocks,
                                                 const BlockHdr* __restrict__ blocks,
                                                 const double* __restrict__ pairs,
                                                 const double* __restrict__ f,
+                                                const double* __restrict__ gm,
                                                 double* __restrict__ partial, int Mv, int Mw) {
   __shared__ double fb[D * 64];
+  __shared__ double gb[MIX ? D * 64 : 1];
   const WorkItem it = items[blockIdx.x];
   if (it.chunk < 0) return;  // padding slot of the XCD-aware order
   const Group g = groups[it.group];
@@ -60,11 +65,17 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
     int64_t pf = 0;
     const double* fx = f + (int64_t)xc * D;
     const double* fy = f + (int64_t)yc * D;
-    double fa[D];
+    double fa[D], ga[MIX ? D : 1];
 #pragma unroll
     for (int p = 0; p < D; ++p) fa[p] = fx[p];
 #pragma unroll
     for (int q = 0; q < D; ++q) fb[q * 64 + lane] = fy[q];
+    if constexpr (MIX) {
+#pragma unroll
+      for (int p = 0; p < D; ++p) ga[p] = gm[(int64_t)xc * D + p];
+#pragma unroll
+      for (int q = 0; q < D; ++q) gb[q * 64 + lane] = gm[(int64_t)yc * D + q];
+    }
 #pragma unroll
     for (int p = 0; p < D; ++p) {
       const int j0 = B->seg[p], j1 = B->seg[p + 1];
@@ -76,14 +87,16 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
         pf += kWin;
       }
       const double fap = fa[p];
+      const double gap = MIX ? ga[p] : 0.0;
       // q of the next pair is read together with the current coefficients,
       // so each pair costs one scalar-load round trip
       int qn = j0 < j1 ? (int)(__double_as_longlong(P[(int64_t)j0 * (D + 1) + D]) & 255) : 0;
       for (int j = j0; j < j1; ++j) {
         const double* R = P + (int64_t)j * (D + 1);
         const double fbq = fb[qn * 64 + lane];
+        const double gbq = MIX ? gb[qn * 64 + lane] : 0.0;
         if (j + 1 < j1) qn = (int)(__double_as_longlong(R[(D + 1) + D]) & 255);
-        const double gq = fap * fbq;
+        const double gq = MIX ? __dadd_rn(__dmul_rn(gap, fbq), __dmul_rn(fap, gbq)) : fap * fbq;
 #pragma unroll
         for (int l = 0; l < D; ++l) acc[l] = __fma_rn(R[l], gq, acc[l]);
       }
@@ -112,16 +125,18 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
 // j + 16 (i%4), element i/4 (the FP64 output layout).
 typedef double dgv_d4 __attribute__((ext_vector_type(4)));
 
-template <int D>
+template <int D, bool MIX>
 __global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ items,
                                                 const Group* __restrict__ groups,
                                                 const int32_t* __restrict__ gblocks,
                                                 const BlockHdr* __restrict__ blocks,
                                                 const double* __restrict__ pairs,
                                                 const double* __restrict__ f,
+                                                const double* __restrict__ gm,
                                                 double* __restrict__ partial, int Mv, int Mw) {
   constexpr int RT = (D + 15) / 16;
   __shared__ double lfa[D * 64], lfb[D * 64];
+  __shared__ double lga[MIX ? D * 64 : 1], lgb[MIX ? D * 64 : 1];
   const WorkItem it = items[blockIdx.x];
   if (it.chunk < 0) return;
   const Group g = groups[it.group];
@@ -150,6 +165,12 @@ __global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ ite
     for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
 #pragma unroll
     for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
+    if constexpr (MIX) {
+#pragma unroll
+      for (int p = 0; p < D; ++p) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
+#pragma unroll
+      for (int q = 0; q < D; ++q) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
+    }
     __syncthreads();
     const double* P = pairs + B->pbeg * (D + 1);
     const int n = B->npairs;
@@ -174,7 +195,14 @@ __global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ ite
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) {
         const int cell = 16 * ct + jj;
-        const double b = c0 >= 0 ? lfa[p * 64 + cell] * lfb[q * 64 + cell] : 0.0;
+        double b = 0.0;
+        if (c0 >= 0) {
+          if constexpr (MIX)
+            b = __dadd_rn(__dmul_rn(lga[p * 64 + cell], lfb[q * 64 + cell]),
+                          __dmul_rn(lfa[p * 64 + cell], lgb[q * 64 + cell]));
+          else
+            b = lfa[p * 64 + cell] * lfb[q * 64 + cell];
+        }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
           acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rt], b, acc[rt][ct], 0, 0, 0);
@@ -283,7 +311,8 @@ __global__ __launch_bounds__(256) void k_rec_eval(int D, int Mu, int Mv, int Mw,
   }
 }
 
-static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma) {
+// d_g != nullptr: the mixed term with g (EvalCollisionPeriodicMixedTermsA_DGV)
+static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, const double* d_g = nullptr) {
   const Grid& g = ctx->g;
   const PairBlock& pb = ctx->pb;
   const Plan& P = ctx->plan;
@@ -291,11 +320,16 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma) {
   if (nitems == 0) return DGV_OK;
   if (mfma) {
     switch (g.D) {
-#define DGV_MF_CASE(DD)                                                                          \
-  case DD:                                                                                       \
-    hipLaunchKernelGGL(k_pb_mfma<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
-                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
-                       g.Mv, g.Mw);                                                             \
+#define DGV_MF_CASE(DD)                                                                               \
+  case DD:                                                                                            \
+    if (d_g)                                                                                          \
+      hipLaunchKernelGGL((k_pb_mfma<DD, true>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,    \
+                         pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_g,           \
+                         P.d_partial, g.Mv, g.Mw);                                                    \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_pb_mfma<DD, false>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,   \
+                         pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_f,           \
+                         P.d_partial, g.Mv, g.Mw);                                                    \
     break;
       DGV_MF_CASE(8)
       DGV_MF_CASE(27)
@@ -307,11 +341,16 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma) {
     return DGV_OK;
   }
   switch (g.D) {
-#define DGV_PB_CASE(DD)                                                                          \
-  case DD:                                                                                       \
-    hipLaunchKernelGGL(k_pb_eval<DD>, dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,         \
-                       pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, P.d_partial, \
-                       g.Mv, g.Mw);                                                             \
+#define DGV_PB_CASE(DD)                                                                               \
+  case DD:                                                                                            \
+    if (d_g)                                                                                          \
+      hipLaunchKernelGGL((k_pb_eval<DD, true>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,    \
+                         pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_g,           \
+                         P.d_partial, g.Mv, g.Mw);                                                    \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_pb_eval<DD, false>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,   \
+                         pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_f,           \
+                         P.d_partial, g.Mv, g.Mw);                                                    \
     break;
     DGV_PB_CASE(1)
     DGV_PB_CASE(8)
@@ -377,6 +416,20 @@ int launch_eval_main_only(dgv_ctx* ctx, const double* d_f) {
   const int k = resolve_kernel(ctx);
   if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, ctx->d_q, 1.0);
   return launch_main_pairblock(ctx, d_f, k == DGV_KERNEL_MFMA);
+}
+
+// EvalCollisionPeriodicMixedTermsA_DGV(f, fm, fc) on the device (pair-block kernels)
+int eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* d_q) {
+  if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set (dgv_ctx_set_aarray / generate)");
+  int k = resolve_kernel(ctx);
+  if (k == DGV_KERNEL_RECORD) {
+    if (!pairblock_supported(ctx->g.D))
+      return set_error(ctx, DGV_ERR_UNSUPPORTED, "the mixed term needs 1, 8 or 27 nodes per cell");
+    k = DGV_KERNEL_PAIRBLOCK;
+  }
+  int rc = launch_main_pairblock(ctx, d_f, k == DGV_KERNEL_MFMA, d_fm);
+  if (rc) return rc;
+  return launch_reduce(ctx, d_q, 1.0);
 }
 
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {

@@ -131,6 +131,11 @@ int dgv_eval_q(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
 int dgv_eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_fc, int64_t n);
 /* EvalCollisionPeriodicA_DGV(f,fc) (collision_mod.f90:29-129): same quantity */
 int dgv_eval_q_a(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
+/* EvalCollisionPeriodicMixedTermsA_DGV(f,fm,fc) (collision_mod.f90:244-332):
+ * fc_i = 2/w_i sum A (fm(xi) f(xi1) + f(xi) fm(xi1)) over the shifted records,
+ * the cross term of the decomposition f = fm + df; equals twice Q(f) for fm = f */
+int dgv_eval_mixed(dgv_ctx* ctx, const double* f, const double* fm, double* fc, int64_t n);
+int dgv_eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* d_fc, int64_t n);
 /* selects the kernel used by dgv_eval_q* (DGV_KERNEL_*) */
 int dgv_ctx_set_kernel(dgv_ctx* ctx, int kernel);
 /* tunables of the pair-block kernel (pairs per work item); 0 = keep */

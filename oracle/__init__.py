@@ -21,6 +21,9 @@ function               reference routine restated
 ``rk_step``            SpatialOperSH_DGV run_mode 0 + TimeIntegratorORK_SH_DGV
                        (spat_oper_mod.f90:28-68, time_integr_mod.f90:26-152)
 ``ab_step``            TimeIntegratorMTS_SH_DGV (time_integr_mod.f90:314-364)
+``eval_mixed``          EvalCollisionPeriodicMixedTermsA_DGV (collision_mod.f90:244-332)
+``prepare_fma``         PrepareFMA_DGV (collision_mod.f90:377-464)
+``eval_linear``         EvalCollisionLinear (collision_mod.f90:346-359)
 ``trunc_threshold``     TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362)
 ``trunc_radius``        TruncAarrsRadius_DGV (dgvtools_mod.f90:1375-1429)
 ``seta`` / ``seta_raw``  SetA_DGV with the Lagrange basis restored, its adaptive
@@ -124,6 +127,11 @@ def lib():
         L.orc_seta.argtypes = [C.POINTER(_Geom), C.c_int, C.c_int32, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_int64, _f64p, _i32p, _i32p, C.c_int]
         L.orc_seta.restype = C.c_int64
+        L.orc_eval_mixed.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p, _f64p,
+                                     _f64p, C.c_int]
+        L.orc_prepare_fma.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p, _f64p]
+        L.orc_eval_linear.argtypes = [C.c_int64, _f64p, _f64p, _f64p]
+        L.orc_eval_linear.restype = None
         L.orc_trunc_threshold.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p, C.c_double]
         L.orc_trunc_threshold.restype = C.c_int64
         L.orc_trunc_radius.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, C.c_double]
@@ -339,3 +347,29 @@ def trunc_radius(G: "Geometry", capphi, A, A_xi, A_xi1, rad):
     a, x, y = np.array(A, np.float64), np.array(A_xi, np.int32), np.array(A_xi1, np.int32)
     n = lib().orc_trunc_radius(G.ptr, cap, a, x, y, rad)
     return cap, a[:n].copy(), x[:n].copy(), y[:n].copy()
+
+
+def eval_mixed(G: "Geometry", capphi, A, A_xi, A_xi1, f, fm, nthreads=1):
+    """EvalCollisionPeriodicMixedTermsA_DGV(f, fm)."""
+    fc = np.zeros(G.nnodes)
+    lib().orc_eval_mixed(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                         _c(A_xi1, np.int32), _c(f, np.float64), _c(fm, np.float64), fc, nthreads)
+    return fc
+
+
+def prepare_fma(G: "Geometry", capphi, A, A_xi, A_xi1, fm):
+    """PrepareFMA_DGV: the dense fmA as an (N, N) array indexed [row, column]."""
+    N = G.nnodes
+    buf = np.zeros(N * N)
+    lib().orc_prepare_fma(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                          _c(A_xi1, np.int32), _c(fm, np.float64), buf)
+    return buf.reshape(N, N).T
+
+
+def eval_linear(fmA, f):
+    """EvalCollisionLinear: fc = 2 f . fmA (fmA indexed [row, column])."""
+    N = fmA.shape[0]
+    colmajor = np.ascontiguousarray(fmA.T).reshape(-1)
+    fc = np.zeros(N)
+    lib().orc_eval_linear(N, colmajor, _c(f, np.float64), fc)
+    return fc

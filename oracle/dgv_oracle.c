@@ -260,6 +260,68 @@ int orc_eval_aplus(const orc_geom* G, const int32_t* capphi, const double* A,
   return 0;
 }
 
+int orc_eval_mixed(const orc_geom* G, const int32_t* capphi, const double* A,
+                   const int32_t* A_xi, const int32_t* A_xi1, const double* f, const double* fm,
+                   double* fc, int nthreads) {
+  int64_t* ashift = (int64_t*)malloc(sizeof(int64_t) * G->nnodes);
+  orc_ashift(G, capphi, ashift);
+  int nt = set_threads(nthreads);
+  /* EvalCollisionPeriodicMixedTermsA_DGV: fc=0, per node the shifted records,
+   * fc += (fm(xi) f(xi1) + f(xi) fm(xi1)) A, then fc = 2 fc / w (lines 283-328) */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t iphi = 0; iphi < G->nnodes; iphi++) {
+    int64_t Ashift = ashift[iphi];
+    int32_t phicap = capphi[G->nodes_phican[iphi] - 1];
+    int dui = G->nodes_dui[iphi], dvi = G->nodes_dvi[iphi], dwi = G->nodes_dwi[iphi];
+    double acc = 0.0;
+    for (int32_t j = 1; j <= phicap; j++) {
+      int64_t k1, k2;
+      if (shift_record(G, A_xi[Ashift + j - 1], A_xi1[Ashift + j - 1], dui, dvi, dwi, &k1, &k2))
+        acc = acc + (fm[k1 - 1] * f[k2 - 1] + f[k1 - 1] * fm[k2 - 1]) * A[Ashift + j - 1];
+    }
+    fc[iphi] = 2 * acc / G->nodes_gwts[iphi];
+  }
+  free(ashift);
+  return 0;
+}
+
+int orc_prepare_fma(const orc_geom* G, const int32_t* capphi, const double* A,
+                    const int32_t* A_xi, const int32_t* A_xi1, const double* fm, double* fmA) {
+  /* PrepareFMA_DGV (collision_mod.f90:377-464): column iphi of the dense N x N
+   * fmA (column-major, fmA[row + N col]) collects fm(xi1) A at row xi and
+   * fm(xi) A at row xi1 over the node's shifted records, then is divided by w */
+  const int64_t N = G->nnodes;
+  int64_t* ashift = (int64_t*)malloc(sizeof(int64_t) * N);
+  orc_ashift(G, capphi, ashift);
+  memset(fmA, 0, sizeof(double) * N * N);
+  for (int64_t iphi = 0; iphi < N; iphi++) {
+    double* col = fmA + iphi * N;
+    int64_t Ashift = ashift[iphi];
+    int32_t phicap = capphi[G->nodes_phican[iphi] - 1];
+    int dui = G->nodes_dui[iphi], dvi = G->nodes_dvi[iphi], dwi = G->nodes_dwi[iphi];
+    for (int32_t j = 1; j <= phicap; j++) {
+      int64_t k1, k2;
+      if (shift_record(G, A_xi[Ashift + j - 1], A_xi1[Ashift + j - 1], dui, dvi, dwi, &k1, &k2)) {
+        col[k1 - 1] = col[k1 - 1] + fm[k2 - 1] * A[Ashift + j - 1];
+        col[k2 - 1] = col[k2 - 1] + fm[k1 - 1] * A[Ashift + j - 1];
+      }
+    }
+    for (int64_t r = 0; r < N; r++) col[r] = col[r] / G->nodes_gwts[iphi];
+  }
+  free(ashift);
+  return 0;
+}
+
+void orc_eval_linear(int64_t N, const double* fmA, const double* f, double* fc) {
+  /* EvalCollisionLinear (collision_mod.f90:346-359): fc = 2 MATMUL(f, fmA) */
+  for (int64_t i = 0; i < N; i++) {
+    const double* col = fmA + i * N;
+    double s = 0.0;
+    for (int64_t r = 0; r < N; r++) s = s + f[r] * col[r];
+    fc[i] = 2 * s;
+  }
+}
+
 int orc_eval_aplus_list(const orc_geom* G, const int32_t* capphi, const double* A,
                         const int32_t* A_xi, const int32_t* A_xi1, const double* f, double* fc,
                         const int64_t* nodes, int64_t nsel, int nthreads) {

@@ -110,6 +110,15 @@ int64_t orc_seta(const orc_geom* G, int mode, int32_t i1, double Trad,
                  double ErrChi, double ErrEps, double min_sens, int64_t cap,
                  double* A, int32_t* A_xi, int32_t* A_xi1, int nthreads);
 
+/* EvalCollisionPeriodicMixedTermsA_DGV (collision_mod.f90:244-332)           */
+int orc_eval_mixed(const orc_geom* G, const int32_t* capphi, const double* A,
+                   const int32_t* A_xi, const int32_t* A_xi1, const double* f, const double* fm,
+                   double* fc, int nthreads);
+/* PrepareFMA_DGV (collision_mod.f90:377-464): dense fmA, column-major N x N   */
+int orc_prepare_fma(const orc_geom* G, const int32_t* capphi, const double* A,
+                    const int32_t* A_xi, const int32_t* A_xi1, const double* fm, double* fmA);
+/* EvalCollisionLinear (collision_mod.f90:346-359): fc = 2 f . fmA            */
+void orc_eval_linear(int64_t N, const double* fmA, const double* f, double* fc);
 /* TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362): keep records with
  * |A| > trhld, in order; capphi updated in place.  Returns the new count.     */
 int64_t orc_trunc_threshold(int64_t ncap, int32_t* capphi, double* A, int32_t* A_xi,
