@@ -89,7 +89,12 @@ def main():
     ap.add_argument("--sur-rows", type=int, default=512, help="(f, Q) pairs generated for the surrogate")
     ap.add_argument("--sur-steps", type=int, default=300)
     ap.add_argument("--profile", action="store_true", help="evaluator loops only (for rocprofv3)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only a plain device copy of 1 GiB, 5 times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the evaluator has no multi-GPU path "
               "(how it would shard is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -277,7 +282,8 @@ def main():
             "speedup_fp32comp_batched_vs_fp64_plain": value / single_qps,
         },
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
+                     "traffic": measured_traffic("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
                      "kernel": "k_batch_mfma<27,FP32>", "kernel_ms": r32["kernel_ms"],
                      "flops_per_launch": flops,
                      "hbm_bytes_per_launch_algorithmic": 12.0 * R + 16.0 * N * B,
@@ -295,6 +301,35 @@ def main():
     d.device_free(d_f)
     d.device_free(d_q)
     d.close()
+
+
+def copy_probe():
+    """A copy kernel of known size: 2^28 float32 (1 GiB read, 1 GiB written) per
+    launch, for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy probe: 5 copies of 1 GiB")
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/hbm_traffic.txt, key=value lines), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        kv = dict(t.split("=", 1) for t in line.split() if "=" in t)
+        if kv.get("kernel") == kernel and kv.get("workload") == workload:
+            try:
+                return float(kv["fetch_kib"]) * 1024 / float(kv["copy_fetch_factor"]) + \
+                    float(kv["write_kib"]) * 1024 / float(kv["copy_write_factor"])
+            except (KeyError, ValueError):
+                return None
+    return None
 
 
 def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):

@@ -25,6 +25,7 @@ arithmetic runs in libdgv_amd.so on the GPU; there is no host fallback.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -113,7 +114,15 @@ class DGV:
         self.e_time = 0.0
 
     # -- lifetime ----------------------------------------------------------------
+    def _register_dependent(self, obj):
+        """Batches and surrogates built on this context; closed before it."""
+        if not hasattr(self, "_dependents"):
+            self._dependents = weakref.WeakSet()
+        self._dependents.add(obj)
+
     def close(self):
+        for obj in list(getattr(self, "_dependents", ())):
+            obj.close()
         if getattr(self, "_ctx", None):
             self._L.dgv_ctx_destroy(self._ctx)
             self._ctx = None
@@ -328,23 +337,23 @@ class DGV:
         return p.value
 
     def device_free(self, ptr: int):
-        self._chk(self._L.dgv_device_free(self._ctx, C.c_void_p(ptr)))
+        self._chk(self._L.dgv_device_free(self._ctx, C.c_void_p(int(ptr))))
 
     def htod(self, dst: int, a: np.ndarray):
         a = np.ascontiguousarray(a)
-        self._chk(self._L.dgv_memcpy_htod(self._ctx, C.c_void_p(dst), a.ctypes.data_as(C.c_void_p),
+        self._chk(self._L.dgv_memcpy_htod(self._ctx, C.c_void_p(int(dst)), a.ctypes.data_as(C.c_void_p),
                                           a.nbytes))
 
     def dtoh(self, a: np.ndarray, src: int):
-        self._chk(self._L.dgv_memcpy_dtoh(self._ctx, a.ctypes.data_as(C.c_void_p), C.c_void_p(src),
+        self._chk(self._L.dgv_memcpy_dtoh(self._ctx, a.ctypes.data_as(C.c_void_p), C.c_void_p(int(src)),
                                           a.nbytes))
 
     def eval_q_dev(self, d_f: int, d_q: int):
-        self._chk(self._L.dgv_eval_q_dev(self._ctx, C.c_void_p(d_f), C.c_void_p(d_q), self.nnodes))
+        self._chk(self._L.dgv_eval_q_dev(self._ctx, C.c_void_p(int(d_f)), C.c_void_p(int(d_q)), self.nnodes))
 
     def rk_run_dev(self, d_f: int, dt: float, nsteps: int, rkmts: Optional[int] = None,
                    use_graph: bool = True):
-        self._chk(self._L.dgv_rk_run_dev(self._ctx, C.c_void_p(d_f), self.nnodes, dt, self.coef,
+        self._chk(self._L.dgv_rk_run_dev(self._ctx, C.c_void_p(int(d_f)), self.nnodes, dt, self.coef,
                                          rkmts or self.rkmts, nsteps, int(use_graph)))
 
     def synchronize(self):
@@ -353,7 +362,7 @@ class DGV:
     def time_eval_q_dev(self, d_f: int, d_q: int, niter: int):
         a = C.c_double()
         b = C.c_double()
-        self._chk(self._L.dgv_time_eval_q_dev(self._ctx, C.c_void_p(d_f), C.c_void_p(d_q), niter,
+        self._chk(self._L.dgv_time_eval_q_dev(self._ctx, C.c_void_p(int(d_f)), C.c_void_p(int(d_q)), niter,
                                               C.byref(a), C.byref(b)))
         return a.value, b.value
 
@@ -372,6 +381,8 @@ class DGVBatch:
         check(self._L.dgv_batch_create(arr, len(self.grids), iptr(self.grid_of_instance),
                                        len(self.grid_of_instance), precision, C.byref(p)))
         self._b = p
+        for g in self.grids:
+            g._register_dependent(self)
         self.ninstances = len(self.grid_of_instance)
 
     def close(self):

@@ -16,8 +16,9 @@
 // i.e. the same contraction as the single-VDF kernel with instances in place
 // of cells.  FP64: v_mfma_f64_16x16x4_f64.  FP32-compensated: the coefficients
 // and f are rounded to FP32, products and the per-block sums run on
-// v_mfma_f32_16x16x4_f32, and the block sums are folded into an FP32 (hi, lo)
-// pair with the two-sum of Knuth; the item writes hi + lo in FP64.  A second
+// v_mfma_f32_16x16x4_f32, and the sums of every 32 pairs are folded into an
+// FP32 (hi, lo) pair with the two-sum of Knuth; the item writes hi + lo in FP64.
+// FP64: two waves per item, block sums added into an FP64 running total.  A second
 // kernel adds the items of each (cell, instance) in a fixed order in FP64 and
 // applies 2/w (and the spatial operator's scale).
 #include <algorithm>
@@ -145,20 +146,28 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bp)) + (b - bp);
 }
 
-template <int D, bool FP32>
-__global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
-                                                   const BatchGridDev* __restrict__ grids,
-                                                   const double* __restrict__ f64, const float* __restrict__ f32,
-                                                   double* __restrict__ partial, int* __restrict__ counter) {
+// NW waves per item, each owning 64 / NW of the chunk's instances (CT column
+// tiles of 16).  FP32: NW = 1, the matrix-core sums of every 32 pairs folded
+// into an FP32 (hi, lo) pair.  FP64: NW = 2, per-block sums added into an FP64 running total, so no
+// single accumulator carries a long sequential sum (the registers of the
+// total fit beside the matrix-core accumulators at two waves per SIMD).
+template <int D, bool FP32, int NW>
+__global__ __launch_bounds__(64 * NW) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
+                                                        const BatchGridDev* __restrict__ grids,
+                                                        const double* __restrict__ f64,
+                                                        const float* __restrict__ f32,
+                                                        double* __restrict__ partial, int* __restrict__ counter) {
   constexpr int RT = (D + 15) / 16;
+  constexpr int CT = 4 / NW;
   using T = typename std::conditional<FP32, float, double>::type;
   using V4 = typename std::conditional<FP32, bf4, bd4>::type;
   __shared__ T lfa[D * 64], lfb[D * 64];
   __shared__ int s_item;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kk = lane >> 4, jj = lane & 15;
+  const int col0 = wave * 16 * CT;   // this wave's first instance column in the chunk
   for (;;) {
-    if (lane == 0) s_item = atomicAdd(counter, 1);
+    if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
     const int id = s_item;
     __syncthreads();
@@ -168,15 +177,15 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
     const int Mv = G.Mv, Mw = G.Mw;
     const int c = it.cell;
     const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
-    const int64_t vb = (int64_t)it.vc * 64 + lane;   // this lane's instance (staging)
-    V4 acc[RT][4], hi[RT][4], lo[RT][4];
+    const int64_t vb = (int64_t)it.vc * 64 + lane;   // the instance this thread stages
+    V4 acc[RT][CT], hi[RT][CT], lo[RT][FP32 ? CT : 1];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
+      for (int ct = 0; ct < CT; ++ct) {
         acc[rt][ct] = V4{0, 0, 0, 0};
         hi[rt][ct] = V4{0, 0, 0, 0};
-        lo[rt][ct] = V4{0, 0, 0, 0};
+        if constexpr (FP32) lo[rt][ct] = V4{0, 0, 0, 0};
       }
     for (int bi = it.b0; bi < it.b1; ++bi) {
       const BlockHdr* B = G.blocks + G.blist[bi];
@@ -191,10 +200,8 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
         for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
       } else {
         const double* F = f64 + G.foff;
-#pragma unroll
-        for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
-#pragma unroll
-        for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
+        for (int p = wave; p < D; p += NW) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
+        for (int q = wave; q < D; q += NW) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
       }
       __syncthreads();
       const int n = B->npairs;
@@ -210,6 +217,26 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
         if constexpr (FP32) code = ok ? __float_as_int(R[D]) : -1;
         else code = ok ? (int)__double_as_longlong(R[D]) : -1;
       };
+      // FP32: the matrix-core sums are folded into (hi, lo) every kFold steps of
+      // 4 pairs, so no FP32 accumulator adds more than 4 kFold products
+      auto fold32 = [&]() {
+        if constexpr (FP32) {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                float s2, er;
+                two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
+                hi[rt][ct][e] = s2;
+                lo[rt][ct][e] += er;
+                acc[rt][ct][e] = 0.0f;
+              }
+        }
+      };
+      constexpr int kFold = 8;
+      int nstep = 0;
       T a0[RT], a1[RT];
       int c0, c1;
       load(0, a0, c0);
@@ -220,8 +247,8 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
         load(s + 8, a2, c2);
         const int p = c0 >> 8, q = c0 & 255;
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-          const int v = 16 * ct + jj;
+        for (int ct = 0; ct < CT; ++ct) {
+          const int v = col0 + 16 * ct + jj;
           const T b = c0 >= 0 ? lfa[p * 64 + v] * lfb[q * 64 + v] : (T)0;
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt) {
@@ -235,27 +262,30 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
         for (int rt = 0; rt < RT; ++rt) { a0[rt] = a1[rt]; a1[rt] = a2[rt]; }
         c0 = c1;
         c1 = c2;
+        if constexpr (FP32) {
+          if (++nstep == kFold) {
+            fold32();
+            nstep = 0;
+          }
+        }
       }
-      if constexpr (FP32) {  // fold the block sum into the compensated accumulator
+      if constexpr (FP32) {  // the block's remaining steps
+        if (nstep > 0) fold32();
+      } else {  // two-level FP64 sum
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              float s, er;
-              two_sum(hi[rt][ct][e], acc[rt][ct][e], s, er);
-              hi[rt][ct][e] = s;
-              lo[rt][ct][e] += er;
-              acc[rt][ct][e] = 0.0f;
-            }
+          for (int ct = 0; ct < CT; ++ct) {
+            hi[rt][ct] += acc[rt][ct];
+            acc[rt][ct] = V4{0, 0, 0, 0};
+          }
       }
     }
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        const int v = 16 * ct + jj;
+      for (int ct = 0; ct < CT; ++ct) {
+        const int v = col0 + 16 * ct + jj;
         double* out = partial + (it.prow + v) * D;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -263,7 +293,7 @@ __global__ __launch_bounds__(64) void k_batch_mfma(const BatchItem* __restrict__
           const int l = FP32 ? 16 * rt + 4 * kk + e : 16 * rt + 4 * e + kk;
           if (l < D) {
             if constexpr (FP32) out[l] = (double)hi[rt][ct][e] + (double)lo[rt][ct][e];
-            else out[l] = acc[rt][ct][e];
+            else out[l] = hi[rt][ct][e];
           }
         }
       }
@@ -490,22 +520,24 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
   const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
-  const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc));
+  const bool fp32 = b->precision == DGV_FP32_COMP;
+  const int nw = fp32 ? 1 : 2;   // waves per workgroup (item)
+  const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc / nw));
   if (nitems > 0) {
     if (D == 27) {
-      if (b->precision == DGV_FP32_COMP)
-        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+      if (fp32)
+        hipLaunchKernelGGL((k_batch_mfma<27, true, 1>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids,
+                           d_f, b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<27, false, 2>), dim3(nwg), dim3(128), 0, st, P.d_items, nitems, b->d_grids,
+                           d_f, b->d_f32, P.d_partial, b->d_counter);
     } else {
-      if (b->precision == DGV_FP32_COMP)
-        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+      if (fp32)
+        hipLaunchKernelGGL((k_batch_mfma<8, true, 1>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids,
+                           d_f, b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<8, false, 2>), dim3(nwg), dim3(128), 0, st, P.d_items, nitems, b->d_grids,
+                           d_f, b->d_f32, P.d_partial, b->d_counter);
     }
   }
   if (reduce && P.nred > 0) {
@@ -628,7 +660,8 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
   if (hipMalloc(&b->d_f, nb) || hipMalloc(&b->d_q, nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
   for (int i = 0; i < 4; ++i)
     if (hipMalloc(&b->d_tmp[i], nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
-  if (hipMemset(b->d_f, 0, nb) || hipMemset(b->d_q, 0, nb)) return fail(DGV_ERR_HIP, "hipMemset");
+  if (hipMemsetAsync(b->d_f, 0, nb, b->stream) || hipMemsetAsync(b->d_q, 0, nb, b->stream))
+    return fail(DGV_ERR_HIP, "hipMemset");
   if (precision == DGV_FP32_COMP && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
     return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
   if (hipMalloc(&b->d_counter, sizeof(int) * 4)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");

@@ -655,10 +655,12 @@ int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t
     const float sd = (l + 1 == nlayers ? 0.1f : 1.0f) / std::sqrt((float)ly.in);
     for (int o = 0; o < ly.out; ++o)
       for (int i = 0; i < ly.in; ++i) W[(size_t)o * ly.inp + i] = sd * nd(rng);
-    if (hipMemcpy(ly.W, W.data(), 4 * nw, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(ly.mW, 0, 4 * nw) || hipMemset(ly.vW, 0, 4 * nw) || hipMemset(ly.G, 0, 4 * nw) ||
-        hipMemset(ly.b, 0, 4 * ly.outp) || hipMemset(ly.mb, 0, 4 * ly.outp) || hipMemset(ly.vb, 0, 4 * ly.outp) ||
-        hipMemset(ly.gb, 0, 4 * ly.outp))
+    hipStream_t st = ctx->stream;   // everything in stream order: the context's stream ignores the null stream
+    if (hipMemcpyAsync(ly.W, W.data(), 4 * nw, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(ly.mW, 0, 4 * nw, st) || hipMemsetAsync(ly.vW, 0, 4 * nw, st) ||
+        hipMemsetAsync(ly.G, 0, 4 * nw, st) || hipMemsetAsync(ly.b, 0, 4 * ly.outp, st) ||
+        hipMemsetAsync(ly.mb, 0, 4 * ly.outp, st) || hipMemsetAsync(ly.vb, 0, 4 * ly.outp, st) ||
+        hipMemsetAsync(ly.gb, 0, 4 * ly.outp, st) || hipStreamSynchronize(st))
       return fail(DGV_ERR_HIP, "surrogate initialisation failed");
     hipLaunchKernelGGL(k_weights_bf16, dim3(ly.inp / 64, ly.outp / 64), dim3(256), 0, ctx->stream, ly.outp, ly.inp,
                        ly.W, ly.Wf, ly.Wb);
@@ -721,13 +723,15 @@ static int layer_copy(dgv_surrogate* s, int32_t layer, float* W, float* b, const
   SHIP(s, hipSetDevice(s->ctx->device));
   SHIP(s, hipStreamSynchronize(s->ctx->stream));
   const size_t rowb = sizeof(float) * ly.in, pitch = sizeof(float) * ly.inp;
+  hipStream_t st = s->ctx->stream;
   if (to_device) {
-    if (W) SHIP(s, hipMemcpy2D(const_cast<float*>(Wd), pitch, W, rowb, rowb, ly.out, hipMemcpyHostToDevice));
-    if (b) SHIP(s, hipMemcpy(const_cast<float*>(bd), b, sizeof(float) * ly.out, hipMemcpyHostToDevice));
+    if (W) SHIP(s, hipMemcpy2DAsync(const_cast<float*>(Wd), pitch, W, rowb, rowb, ly.out, hipMemcpyHostToDevice, st));
+    if (b) SHIP(s, hipMemcpyAsync(const_cast<float*>(bd), b, sizeof(float) * ly.out, hipMemcpyHostToDevice, st));
   } else {
-    if (W) SHIP(s, hipMemcpy2D(W, rowb, Wd, pitch, rowb, ly.out, hipMemcpyDeviceToHost));
-    if (b) SHIP(s, hipMemcpy(b, bd, sizeof(float) * ly.out, hipMemcpyDeviceToHost));
+    if (W) SHIP(s, hipMemcpy2DAsync(W, rowb, Wd, pitch, rowb, ly.out, hipMemcpyDeviceToHost, st));
+    if (b) SHIP(s, hipMemcpyAsync(b, bd, sizeof(float) * ly.out, hipMemcpyDeviceToHost, st));
   }
+  SHIP(s, hipStreamSynchronize(st));
   return DGV_OK;
 }
 
@@ -853,7 +857,9 @@ int dgv_sur_set_probe_cells(dgv_surrogate* s, const int32_t* cells, int32_t ncel
   hipFree(s->d_probe_nodes);
   s->d_probe_nodes = nullptr;
   SHIP(s, hipMalloc(&s->d_probe_nodes, sizeof(int32_t) * nodes.size()));
-  SHIP(s, hipMemcpy(s->d_probe_nodes, nodes.data(), sizeof(int32_t) * nodes.size(), hipMemcpyHostToDevice));
+  SHIP(s, hipMemcpyAsync(s->d_probe_nodes, nodes.data(), sizeof(int32_t) * nodes.size(), hipMemcpyHostToDevice,
+                         s->ctx->stream));
+  SHIP(s, hipStreamSynchronize(s->ctx->stream));
   s->nprobe_nodes = (int)nodes.size();
   s->probe_cells = cl;
   return DGV_OK;

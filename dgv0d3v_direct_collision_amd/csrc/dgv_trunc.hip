@@ -124,7 +124,7 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
     hipFree(d_new);
     return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(truncation map) failed");
   }
-  e = hipMemcpy(d_pos, newpos.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice);
+  e = hipMemcpyAsync(d_pos, newpos.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream);
   const int64_t tot = np * (D + 1);
   if (e == hipSuccess && tot > 0)
     hipLaunchKernelGGL(k_gather_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, np, D,
@@ -139,17 +139,22 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   hipFree(pb.d_pairs);
   pb.d_pairs = d_new;
   pb.npairs = next;
-  DGV_HIP(ctx, hipMemcpy(pb.d_blocks, pb.blocks.data(), sizeof(BlockHdr) * pb.blocks.size(), hipMemcpyHostToDevice));
-  DGV_HIP(ctx, hipMemcpy(pb.d_groups, pb.groups.data(), sizeof(Group) * pb.groups.size(), hipMemcpyHostToDevice));
+  DGV_HIP(ctx, hipMemcpyAsync(pb.d_blocks, pb.blocks.data(), sizeof(BlockHdr) * pb.blocks.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  DGV_HIP(ctx, hipMemcpyAsync(pb.d_groups, pb.groups.data(), sizeof(Group) * pb.groups.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
   // record counts per basis function
   unsigned long long* d_cnt = nullptr;
   DGV_HIP(ctx, hipMalloc(&d_cnt, sizeof(unsigned long long) * 64));
-  DGV_HIP(ctx, hipMemset(d_cnt, 0, sizeof(unsigned long long) * 64));
+  DGV_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 64, ctx->stream));
   if (next > 0)
     hipLaunchKernelGGL(k_count_nonzero, dim3(1024), dim3(256), 0, ctx->stream, next, D, pb.d_pairs, d_cnt);
   std::vector<unsigned long long> cnt(64, 0);
   e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpy(cnt.data(), d_cnt, sizeof(unsigned long long) * 64, hipMemcpyDeviceToHost);
+  // the counts come back on the context's stream (it does not wait for the null stream)
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned long long) * 64, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipFree(d_cnt);
   if (e != hipSuccess) return set_error(ctx, DGV_ERR_HIP, std::string("truncation: ") + hipGetErrorString(e));
   pb.nrec_basis.assign(D, 0);

@@ -33,6 +33,7 @@ class Surrogate:
         p = C.c_void_p()
         check(self._L.dgv_sur_create(dgv.ctx, iptr(self.widths), self.nlayers, batch, seed, C.byref(p)), dgv.ctx)
         self._s = p
+        dgv._register_dependent(self)
         self.f_scale = 1.0
         self.q_scale = 1.0
 
@@ -77,13 +78,13 @@ class Surrogate:
     # -- device-pointer API ---------------------------------------------------------
     def train_step(self, d_f: int, d_q: int, nrows: int, lr: float, ld: Optional[int] = None) -> float:
         loss = C.c_double()
-        self._chk(self._L.dgv_sur_train_step(self._s, C.c_void_p(d_f), C.c_void_p(d_q), ld or self.dgv.nnodes,
+        self._chk(self._L.dgv_sur_train_step(self._s, C.c_void_p(int(d_f)), C.c_void_p(int(d_q)), ld or self.dgv.nnodes,
                                              nrows, lr, C.byref(loss)))
         return loss.value
 
     def predict(self, d_f: int, nrows: int, d_q: int, project: bool = True, ld: Optional[int] = None):
-        self._chk(self._L.dgv_sur_predict(self._s, C.c_void_p(d_f), ld or self.dgv.nnodes, nrows,
-                                          C.c_void_p(d_q), int(project)))
+        self._chk(self._L.dgv_sur_predict(self._s, C.c_void_p(int(d_f)), ld or self.dgv.nnodes, nrows,
+                                          C.c_void_p(int(d_q)), int(project)))
 
     def set_probe_cells(self, cells):
         c = np.ascontiguousarray(cells, np.int32)
@@ -95,14 +96,14 @@ class Surrogate:
         res = np.zeros(nrows)
         fb = np.zeros(nrows, np.int32)
         nfb = C.c_int64()
-        self._chk(self._L.dgv_sur_eval(self._s, C.c_void_p(d_f), ld or self.dgv.nnodes, nrows, C.c_void_p(d_q),
+        self._chk(self._L.dgv_sur_eval(self._s, C.c_void_p(int(d_f)), ld or self.dgv.nnodes, nrows, C.c_void_p(int(d_q)),
                                        tol, res.ctypes.data_as(C.POINTER(C.c_double)), iptr(fb), C.byref(nfb)))
         return res, fb.astype(bool), nfb.value
 
     def time_predict(self, d_f: int, nrows: int, d_q: int, niter: int = 5, ld: Optional[int] = None) -> float:
         ms = C.c_double()
-        self._chk(self._L.dgv_sur_time_predict(self._s, C.c_void_p(d_f), ld or self.dgv.nnodes, nrows,
-                                               C.c_void_p(d_q), niter, C.byref(ms)))
+        self._chk(self._L.dgv_sur_time_predict(self._s, C.c_void_p(int(d_f)), ld or self.dgv.nnodes, nrows,
+                                               C.c_void_p(int(d_q)), niter, C.byref(ms)))
         return ms.value
 
     # -- host-array conveniences (small sizes, tests) --------------------------------
