@@ -160,8 +160,11 @@ def main():
         b = dgv.DGVBatch([d], which, getattr(dgv, "DGV_" + prec))
         for i in range(B):
             b.set_f(i, fs[i])
-        tb, tms = b.autotune()
-        log(f"batch {prec}: autotune {tb} pairs/segment ({tms:.1f} ms)")
+        if prec == "FP32_COMP":
+            tb, tms = b.autotune()
+            log(f"batch {prec}: autotune {tb} pairs/segment ({tms:.1f} ms)")
+        else:   # the FP32 path's segment size
+            b.set_tile(tb)
         for _ in range(args.warmup):
             b.eval_q()
         t2 = time.perf_counter()
@@ -341,7 +344,7 @@ def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
     n = args.sur_rows
     nte = max(32, n // 8)
     ntr = n - nte
-    log(f"surrogate: generating {n} (f, Q) pairs with the batched evaluator")
+    log(f"surrogate: generating {n} (f, Q) pairs with the batched FP32-compensated evaluator")
     fs = np.stack([mixture_vdf(u, v, w, gw, rng, kmax=4) if i % 2 else config_vdf(u, v, w, gw, M, rng)
                    for i in range(n)])
     qs = np.zeros_like(fs)
@@ -362,6 +365,7 @@ def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
     d_p = d.device_alloc(fs.nbytes)
     d.htod(d_f, fs)
     d.htod(d_q, qs)
+    log(f"surrogate: {gen_s:.1f} s for the pairs; creating the network")
     s = dgv.Surrogate(d, [N, 2048, 2048, 2048, N], batch=256, seed=11)
     s.set_scales(float(np.sqrt(np.mean(fs[:ntr] ** 2))), float(np.sqrt(np.mean(qs[:ntr] ** 2))))
     tt = time.perf_counter()
@@ -369,6 +373,8 @@ def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
     for it in range(args.sur_steps):
         r0 = int(rng.integers(0, max(1, ntr - 256 + 1)))
         losses.append(s.train_step(d_f + 8 * N * r0, d_q + 8 * N * r0, min(256, ntr), 1e-4))
+        if it % 100 == 0:
+            log(f"surrogate: step {it} loss {losses[-1]:.4e}")
     d.synchronize()
     train_s = time.perf_counter() - tt
     te = d_f + 8 * N * ntr

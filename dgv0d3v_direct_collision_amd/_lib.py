@@ -96,6 +96,8 @@ SIGNATURES = {
     "dgv_batch_eval_q": (C.c_int, [_vp]),
     "dgv_batch_rk_run": (C.c_int, [_vp, _dbl, _dbl, _i32, _i64, C.c_int]),
     "dgv_batch_autotune": (C.c_int, [_vp, _ip, _dp]),
+    "dgv_batch_set_tile": (C.c_int, [_vp, _i32, _i32]),
+    "dgv_batch_set_projection": (C.c_int, [_vp, _i32]),
     "dgv_batch_device_buffers": (C.c_int, [_vp, _pp, _pp]),
     "dgv_batch_project_invariants": (C.c_int, [_vp]),
     "dgv_batch_set_probe_cells": (C.c_int, [_vp, _ip, _i32]),

@@ -446,6 +446,14 @@ class DGVBatch:
         self._chk(self._L.dgv_batch_autotune(self._b, C.byref(best), C.byref(ms)))
         return best.value, ms.value
 
+    def set_tile(self, pairs_per_segment: int, waves_per_cu: int = 0):
+        self._chk(self._L.dgv_batch_set_tile(self._b, pairs_per_segment, waves_per_cu))
+
+    def set_projection(self, on: bool):
+        """Project every evaluation's Q onto the complement of the collision
+        invariants (FP64), also inside the RK stages."""
+        self._chk(self._L.dgv_batch_set_projection(self._b, int(on)))
+
     def time_eval(self, niter: int = 3, main_kernel: bool = False):
         """Milliseconds per batched evaluation (HIP events on the batch's
         stream); with main_kernel, also those of the persistent kernel alone."""

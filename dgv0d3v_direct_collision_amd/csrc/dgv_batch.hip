@@ -41,10 +41,13 @@ struct BatchGridDev {
   int32_t Bpad, Mv, Mw, D;
 };
 
+// Work item: one cell's segment of blocks for up to kChunksPerItem chunks of
+// 64 instances (vc0 .. vc0 + nvc - 1); chunk k of the item writes the partial
+// rows prow + 64 k .. prow + 64 k + 63.
 struct BatchItem {
   int32_t grid, cell, b0, b1;  // blist range of the segment
-  int32_t vc, pad;             // chunk of 64 instances
-  int64_t prow;                // first partial row (64 rows per item)
+  int32_t vc0, nvc;            // chunks of 64 instances
+  int64_t prow;                // first partial row
 };
 
 // One work decomposition: the items of the queue and the fixed-order reduction
@@ -96,6 +99,7 @@ struct dgv_batch {
   dgv::BatchProjGrid* d_proj = nullptr;   // projection tables per grid
   int32_t* d_proj_task = nullptr;         // (grid, chunk) per block
   int32_t nproj_task = 0;
+  int project = 0;                        // project every evaluation's Q (FP64)
   int* d_counter = nullptr;
   int ncu = 256;
   hipStream_t stream = nullptr;
@@ -146,26 +150,39 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bp)) + (b - bp);
 }
 
-// NW waves per item, each owning 64 / NW of the chunk's instances (CT column
-// tiles of 16).  FP32: NW = 1, the matrix-core sums of every 32 pairs folded
-// into an FP32 (hi, lo) pair.  FP64: NW = 2, per-block sums added into an FP64 running total, so no
-// single accumulator carries a long sequential sum (the registers of the
-// total fit beside the matrix-core accumulators at two waves per SIMD).
-template <int D, bool FP32, int NW>
-__global__ __launch_bounds__(64 * NW) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
-                                                        const BatchGridDev* __restrict__ grids,
-                                                        const double* __restrict__ f64,
-                                                        const float* __restrict__ f32,
-                                                        double* __restrict__ partial, int* __restrict__ counter) {
+// One workgroup of four waves per item; all waves walk the same blocks and
+// pairs in step (a barrier per block), so the coefficient stream of a segment
+// is fetched from memory once per item and served from the caches to the
+// other waves.  FP32: wave w owns chunk w (64 instances, four column tiles of
+// 16); the matrix-core sums of every 32 pairs are folded into an FP32
+// (hi, lo) pair with two-sum.  FP64: waves 2c and 2c+1 share chunk c (32
+// instances each); block sums are added into an FP64 running total.
+constexpr int kWavesPerItem = 4;
+
+template <bool FP32>
+constexpr int chunks_per_item() { return FP32 ? 4 : 2; }
+
+template <int D, bool FP32>
+__global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
+                                                                   const BatchGridDev* __restrict__ grids,
+                                                                   const double* __restrict__ f64,
+                                                                   const float* __restrict__ f32,
+                                                                   double* __restrict__ partial,
+                                                                   int* __restrict__ counter) {
   constexpr int RT = (D + 15) / 16;
-  constexpr int CT = 4 / NW;
+  constexpr int CH = chunks_per_item<FP32>();    // chunks per item
+  constexpr int WPC = kWavesPerItem / CH;        // waves per chunk
+  constexpr int CT = 4 / WPC;                    // column tiles per wave
+  constexpr int kAhead = 4;                      // steps of coefficient loads in flight
   using T = typename std::conditional<FP32, float, double>::type;
   using V4 = typename std::conditional<FP32, bf4, bd4>::type;
-  __shared__ T lfa[D * 64], lfb[D * 64];
+  __shared__ T lfa[CH][D * 64], lfb[CH][D * 64];
   __shared__ int s_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kk = lane >> 4, jj = lane & 15;
-  const int col0 = wave * 16 * CT;   // this wave's first instance column in the chunk
+  const int ch = wave / WPC;                     // this wave's chunk within the item
+  const int sub = wave % WPC;                    // its part of the chunk's staging / columns
+  const int col0 = sub * 16 * CT;
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
@@ -174,10 +191,11 @@ __global__ __launch_bounds__(64 * NW) void k_batch_mfma(const BatchItem* __restr
     if (id >= nitems) return;
     const BatchItem it = items[id];
     const BatchGridDev G = grids[it.grid];
+    const bool active = ch < it.nvc;
     const int Mv = G.Mv, Mw = G.Mw;
     const int c = it.cell;
     const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
-    const int64_t vb = (int64_t)it.vc * 64 + lane;   // the instance this thread stages
+    const int64_t vb = (int64_t)(it.vc0 + ch) * 64 + lane;   // the instance this thread stages
     V4 acc[RT][CT], hi[RT][CT], lo[RT][FP32 ? CT : 1];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -187,91 +205,91 @@ __global__ __launch_bounds__(64 * NW) void k_batch_mfma(const BatchItem* __restr
         hi[rt][ct] = V4{0, 0, 0, 0};
         if constexpr (FP32) lo[rt][ct] = V4{0, 0, 0, 0};
       }
+    T* la = lfa[ch];
+    T* lb = lfb[ch];
     for (int bi = it.b0; bi < it.b1; ++bi) {
       const BlockHdr* B = G.blocks + G.blist[bi];
       const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
       const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
       __syncthreads();
-      if constexpr (FP32) {
-        const float* F = f32 + G.foff;
+      if (active) {
+        if constexpr (FP32) {
+          const float* F = f32 + G.foff;
 #pragma unroll
-        for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
+          for (int p = 0; p < D; ++p) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
 #pragma unroll
-        for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
-      } else {
-        const double* F = f64 + G.foff;
-        for (int p = wave; p < D; p += NW) lfa[p * 64 + lane] = (T)F[((int64_t)xc * D + p) * G.Bpad + vb];
-        for (int q = wave; q < D; q += NW) lfb[q * 64 + lane] = (T)F[((int64_t)yc * D + q) * G.Bpad + vb];
+          for (int q = 0; q < D; ++q) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
+        } else {
+          const double* F = f64 + G.foff;
+          for (int p = sub; p < D; p += WPC) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+          for (int q = sub; q < D; q += WPC) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
+        }
       }
       __syncthreads();
+      if (!active) continue;
       const int n = B->npairs;
       const T* P;
       if constexpr (FP32) P = G.pairs32 + B->pbeg * (D + 1);
       else P = G.pairs64 + B->pbeg * (D + 1);
-      auto load = [&](int s, T (&a)[RT], int& code) {
-        const int pr = s + kk;
-        const bool ok = pr < n;
-        const T* R = P + (int64_t)pr * (D + 1);
+      // coefficient operand of step s: lane i holds T[pair s + i/16][16 rt + i%16]
+      T ab[kAhead + 1][RT];
+      int cb[kAhead + 1];
+#define BATCH_LOAD(slot, s)                                                                   \
+  do {                                                                                        \
+    const int pr_ = (s) + kk;                                                                 \
+    const bool ok_ = pr_ < n;                                                                 \
+    const T* R_ = P + (int64_t)pr_ * (D + 1);                                                 \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+      ab[slot][rt] = (ok_ && 16 * rt + jj < D) ? R_[16 * rt + jj] : (T)0;                     \
+    if constexpr (FP32) cb[slot] = ok_ ? __float_as_int(R_[D]) : -1;                          \
+    else cb[slot] = ok_ ? (int)__double_as_longlong(R_[D]) : -1;                              \
+  } while (0)
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) a[rt] = (ok && 16 * rt + jj < D) ? R[16 * rt + jj] : (T)0;
-        if constexpr (FP32) code = ok ? __float_as_int(R[D]) : -1;
-        else code = ok ? (int)__double_as_longlong(R[D]) : -1;
-      };
-      // FP32: the matrix-core sums are folded into (hi, lo) every kFold steps of
-      // 4 pairs, so no FP32 accumulator adds more than 4 kFold products
-      auto fold32 = [&]() {
-        if constexpr (FP32) {
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                float s2, er;
-                two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
-                hi[rt][ct][e] = s2;
-                lo[rt][ct][e] += er;
-                acc[rt][ct][e] = 0.0f;
-              }
-        }
-      };
-      constexpr int kFold = 8;
+      for (int k = 0; k < kAhead; ++k) BATCH_LOAD(k, 4 * k);
+      constexpr int kFold = 8;   // FP32: fold every 8 steps (32 pairs)
       int nstep = 0;
-      T a0[RT], a1[RT];
-      int c0, c1;
-      load(0, a0, c0);
-      load(4, a1, c1);
       for (int s = 0; s < n; s += 4) {
-        T a2[RT];
-        int c2;
-        load(s + 8, a2, c2);
-        const int p = c0 >> 8, q = c0 & 255;
+        BATCH_LOAD(kAhead, s + 4 * kAhead);
+        const int code = cb[0];
+        const int p = code >> 8, q = code & 255;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           const int v = col0 + 16 * ct + jj;
-          const T b = c0 >= 0 ? lfa[p * 64 + v] * lfb[q * 64 + v] : (T)0;
+          const T b = code >= 0 ? la[p * 64 + v] * lb[q * 64 + v] : (T)0;
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt) {
             if constexpr (FP32)
-              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt], b, acc[rt][ct], 0, 0, 0);
+              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
             else
-              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rt], b, acc[rt][ct], 0, 0, 0);
+              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
           }
         }
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) { a0[rt] = a1[rt]; a1[rt] = a2[rt]; }
-        c0 = c1;
-        c1 = c2;
+        for (int k = 0; k < kAhead; ++k) {
+          cb[k] = cb[k + 1];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) ab[k][rt] = ab[k + 1][rt];
+        }
         if constexpr (FP32) {
-          if (++nstep == kFold) {
-            fold32();
+          if (++nstep == kFold || s + 4 >= n) {
             nstep = 0;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+              for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  float s2, er;
+                  two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
+                  hi[rt][ct][e] = s2;
+                  lo[rt][ct][e] += er;
+                  acc[rt][ct][e] = 0.0f;
+                }
           }
         }
       }
-      if constexpr (FP32) {  // the block's remaining steps
-        if (nstep > 0) fold32();
-      } else {  // two-level FP64 sum
+#undef BATCH_LOAD
+      if constexpr (!FP32) {  // two-level FP64 sum
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -281,12 +299,13 @@ __global__ __launch_bounds__(64 * NW) void k_batch_mfma(const BatchItem* __restr
           }
       }
     }
+    if (!active) continue;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
         const int v = col0 + 16 * ct + jj;
-        double* out = partial + (it.prow + v) * D;
+        double* out = partial + (it.prow + 64 * ch + v) * D;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           // output layouts: FP64 row 4e + lane/16, FP32 row 4 (lane/16) + e
@@ -434,7 +453,7 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
   if (tile <= 0) tile = 1 << 20;
   P.tile = tile;
   std::vector<BatchItem> items;
-  struct Task { int32_t g, c, vc; std::vector<int32_t> rows; };
+  int64_t nrows = 0;
   std::vector<int32_t> key;
   std::vector<int64_t> rptr(1, 0);
   std::vector<int32_t> rrows;
@@ -476,27 +495,34 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
           acc = 0;
         }
       }
-      for (int vc = 0; vc < nchunk; ++vc) {
-        key.push_back((int32_t)gi); key.push_back((int32_t)c); key.push_back(vc);
+      const int chpi = b->precision == DGV_FP32_COMP ? chunks_per_item<true>() : chunks_per_item<false>();
+      for (int vc0 = 0; vc0 < nchunk; vc0 += chpi) {
+        const int nvc = std::min(chpi, nchunk - vc0);
+        std::vector<int64_t> first;   // first partial row of each segment's item
         for (auto& sg : segs) {
           BatchItem it;
           it.grid = (int32_t)gi; it.cell = (int32_t)c; it.b0 = sg.first; it.b1 = sg.second;
-          it.vc = vc; it.pad = 0;
-          it.prow = (int64_t)items.size() * 64;
-          if (it.prow > INT32_MAX - 64) return berr(b, DGV_ERR_UNSUPPORTED, "too many batch work items");
-          rrows.push_back((int32_t)it.prow);
+          it.vc0 = vc0; it.nvc = nvc;
+          it.prow = nrows;
+          nrows += 64 * nvc;
+          if (nrows > INT32_MAX) return berr(b, DGV_ERR_UNSUPPORTED, "too many batch work items");
+          first.push_back(it.prow);
           items.push_back(it);
         }
-        rptr.push_back((int64_t)rrows.size());
+        for (int k = 0; k < nvc; ++k) {
+          key.push_back((int32_t)gi); key.push_back((int32_t)c); key.push_back(vc0 + k);
+          for (int64_t r : first) rrows.push_back((int32_t)(r + 64 * k));
+          rptr.push_back((int64_t)rrows.size());
+        }
       }
     }
   }
   const int D = b->grids[0]->g.D;
   // keep the partial sums within 4 GiB: coarser segments otherwise
-  if ((int64_t)items.size() * 64 * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30))
+  if (nrows * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30))
     return build_batch_plan(b, P, tile * 2, cells);
   P.items.swap(items);
-  P.nrows = (int64_t)P.items.size() * 64;
+  P.nrows = nrows;
   P.nred = (int64_t)rptr.size() - 1;
   BHIP(b, hipMalloc(&P.d_items, sizeof(BatchItem) * std::max<size_t>(P.items.size(), 1)));
   BHIP(b, hipMemcpy(P.d_items, P.items.data(), sizeof(BatchItem) * P.items.size(), hipMemcpyHostToDevice));
@@ -519,25 +545,26 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
   if (b->precision == DGV_FP32_COMP)
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
+  // workgroups of four waves; waves_per_cu sets the persistent grid's size
   const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
   const bool fp32 = b->precision == DGV_FP32_COMP;
-  const int nw = fp32 ? 1 : 2;   // waves per workgroup (item)
-  const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc / nw));
+  const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc / kWavesPerItem));
   if (nitems > 0) {
+    const dim3 blk(64 * kWavesPerItem);
     if (D == 27) {
       if (fp32)
-        hipLaunchKernelGGL((k_batch_mfma<27, true, 1>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids,
-                           d_f, b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<27, false, 2>), dim3(nwg), dim3(128), 0, st, P.d_items, nitems, b->d_grids,
-                           d_f, b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
     } else {
       if (fp32)
-        hipLaunchKernelGGL((k_batch_mfma<8, true, 1>), dim3(nwg), dim3(64), 0, st, P.d_items, nitems, b->d_grids,
-                           d_f, b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
       else
-        hipLaunchKernelGGL((k_batch_mfma<8, false, 2>), dim3(nwg), dim3(128), 0, st, P.d_items, nitems, b->d_grids,
-                           d_f, b->d_f32, P.d_partial, b->d_counter);
+        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
+                           b->d_f32, P.d_partial, b->d_counter);
     }
   }
   if (reduce && P.nred > 0) {
@@ -548,6 +575,9 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
       hipLaunchKernelGGL(k_batch_reduce<8>, dim3((unsigned)P.nred), dim3(64), 0, st, P.nred, P.d_red_key,
                          P.d_red_ptr, P.d_red_rows, b->d_grids, P.d_partial, scale, d_q);
   }
+  if (reduce && b->project && b->nproj_task > 0 && &P == &b->plan)
+    hipLaunchKernelGGL(k_batch_project, dim3((unsigned)b->nproj_task), dim3(1024), 0, st, b->d_proj, b->d_proj_task,
+                       d_q);
   BHIP(b, hipGetLastError());
   return DGV_OK;
 }
@@ -804,6 +834,24 @@ int dgv_batch_eval_q(dgv_batch* b) {
   if (rc) return rc;
   BHIP(b, hipStreamSynchronize(b->stream));
   return DGV_OK;
+}
+
+int dgv_batch_set_projection(dgv_batch* b, int32_t on) {
+  if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
+  b->project = on ? 1 : 0;
+  if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+  return DGV_OK;
+}
+
+int dgv_batch_set_tile(dgv_batch* b, int32_t pairs_per_segment, int32_t waves_per_cu) {
+  if (!b || pairs_per_segment < 0 || waves_per_cu < 0) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (waves_per_cu > 0) b->waves_per_cu = waves_per_cu;
+  int rc = build_batch_plan(b, b->plan, pairs_per_segment);
+  if (rc) return rc;
+  if (!b->probe_cells.empty()) rc = build_batch_plan(b, b->probe, b->plan.tile, &b->probe_cells);
+  return rc;
 }
 
 int dgv_batch_project_invariants(dgv_batch* b) {

@@ -117,30 +117,35 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
 //   g[pair, cell] = f[cell + a][p] * f[cell + b][q],
 // done with v_mfma_f64_16x16x4_f64 (A = 16 basis functions x 4 pairs,
 // B = 4 pairs x 16 cells, C = 16 x 16): RT row tiles cover the D basis
-// functions (rows >= D are zero), four column tiles the 64 cells.  The
-// coefficients reach the A operand through ordinary vector loads (lane i
-// holds T[pair s + i/16][16 rt + i%16]), issued two steps ahead; g is formed
-// on the VALU from the block's f values staged in LDS.  Operand layouts of
+// functions (rows >= D are zero), column tiles the cells.  Two waves share a
+// work item and its LDS-staged f values; wave w owns cells 32 w .. 32 w + 31
+// (two column tiles).  The coefficients reach the A operand through ordinary
+// vector loads (lane i holds T[pair s + i/16][16 rt + i%16]) issued kAhead
+// steps ahead; g is formed on the VALU from the staged f.  Operand layouts of
 // the instruction: A[i][k] and B[k][j] in lane i|j + 16 k; D[i][j] in lane
 // j + 16 (i%4), element i/4 (the FP64 output layout).
 typedef double dgv_d4 __attribute__((ext_vector_type(4)));
 
+constexpr int kPbWaves = 2;
+
 template <int D, bool MIX>
-__global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ items,
-                                                const Group* __restrict__ groups,
-                                                const int32_t* __restrict__ gblocks,
-                                                const BlockHdr* __restrict__ blocks,
-                                                const double* __restrict__ pairs,
-                                                const double* __restrict__ f,
-                                                const double* __restrict__ gm,
-                                                double* __restrict__ partial, int Mv, int Mw) {
+__global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __restrict__ items,
+                                                           const Group* __restrict__ groups,
+                                                           const int32_t* __restrict__ gblocks,
+                                                           const BlockHdr* __restrict__ blocks,
+                                                           const double* __restrict__ pairs,
+                                                           const double* __restrict__ f,
+                                                           const double* __restrict__ gm,
+                                                           double* __restrict__ partial, int Mv, int Mw) {
   constexpr int RT = (D + 15) / 16;
+  constexpr int CT = 4 / kPbWaves;
+  constexpr int kAhead = 4;
   __shared__ double lfa[D * 64], lfb[D * 64];
   __shared__ double lga[MIX ? D * 64 : 1], lgb[MIX ? D * 64 : 1];
   const WorkItem it = items[blockIdx.x];
   if (it.chunk < 0) return;
   const Group g = groups[it.group];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cl = it.chunk * 64 + lane;
   int r = cl < g.ncells ? cl : 0;   // idle lanes stage the box's first cell (in range)
   const int cw = g.lo[2] + r % g.ext[2];
@@ -149,52 +154,49 @@ __global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ ite
   r /= g.ext[1];
   const int cu = g.lo[0] + r;
   const int kk = lane >> 4, jj = lane & 15;
+  const int col0 = wave * 16 * CT;
 
-  dgv_d4 acc[RT][4];
+  dgv_d4 acc[RT][CT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = dgv_d4{0.0, 0.0, 0.0, 0.0};
+    for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = dgv_d4{0.0, 0.0, 0.0, 0.0};
 
   for (int bi = it.bbeg; bi < it.bend; ++bi) {
     const BlockHdr* B = blocks + gblocks[bi];
     const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
     const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
     __syncthreads();  // previous block's LDS reads are done
-#pragma unroll
-    for (int p = 0; p < D; ++p) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
-#pragma unroll
-    for (int q = 0; q < D; ++q) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
+    // thread (wave, lane) stages the nodes p = wave, wave + 2, ... of its lane's cell
+    for (int p = wave; p < D; p += kPbWaves) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
+    for (int q = wave; q < D; q += kPbWaves) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
     if constexpr (MIX) {
-#pragma unroll
-      for (int p = 0; p < D; ++p) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
-#pragma unroll
-      for (int q = 0; q < D; ++q) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
+      for (int p = wave; p < D; p += kPbWaves) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
+      for (int q = wave; q < D; q += kPbWaves) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
     }
     __syncthreads();
     const double* P = pairs + B->pbeg * (D + 1);
     const int n = B->npairs;
-    // operands of step s: this lane's pair is s + kk
-    auto load = [&](int s, double (&a)[RT], int& code) {
-      const int pr = s + kk;
-      const bool ok = pr < n;
-      const double* R = P + (int64_t)pr * (D + 1);
+    double ab[kAhead + 1][RT];
+    int cb[kAhead + 1];
+#define PB_LOAD(slot, s)                                                                  \
+  do {                                                                                    \
+    const int pr_ = (s) + kk;                                                             \
+    const bool ok_ = pr_ < n;                                                             \
+    const double* R_ = P + (int64_t)pr_ * (D + 1);                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
+      ab[slot][rt] = (ok_ && 16 * rt + jj < D) ? R_[16 * rt + jj] : 0.0;                  \
+    cb[slot] = ok_ ? (int)__double_as_longlong(R_[D]) : -1;                               \
+  } while (0)
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) a[rt] = (ok && 16 * rt + jj < D) ? R[16 * rt + jj] : 0.0;
-      code = ok ? (int)__double_as_longlong(R[D]) : -1;
-    };
-    double a0[RT], a1[RT];
-    int c0, c1;
-    load(0, a0, c0);
-    load(4, a1, c1);
+    for (int k = 0; k < kAhead; ++k) PB_LOAD(k, 4 * k);
     for (int s = 0; s < n; s += 4) {
-      double a2[RT];
-      int c2;
-      load(s + 8, a2, c2);
+      PB_LOAD(kAhead, s + 4 * kAhead);
+      const int c0 = cb[0];
       const int p = c0 >> 8, q = c0 & 255;
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        const int cell = 16 * ct + jj;
+      for (int ct = 0; ct < CT; ++ct) {
+        const int cell = col0 + 16 * ct + jj;
         double b = 0.0;
         if (c0 >= 0) {
           if constexpr (MIX)
@@ -205,19 +207,22 @@ __global__ __launch_bounds__(64) void k_pb_mfma(const WorkItem* __restrict__ ite
         }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
-          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rt], b, acc[rt][ct], 0, 0, 0);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
       }
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) { a0[rt] = a1[rt]; a1[rt] = a2[rt]; }
-      c0 = c1;
-      c1 = c2;
+      for (int k = 0; k < kAhead; ++k) {
+        cb[k] = cb[k + 1];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) ab[k][rt] = ab[k + 1][rt];
+      }
     }
+#undef PB_LOAD
   }
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      const int cell = 16 * ct + jj;
+    for (int ct = 0; ct < CT; ++ct) {
+      const int cell = col0 + 16 * ct + jj;
       if (it.chunk * 64 + cell >= g.ncells) continue;
       double* out = partial + (it.prow + cell) * D;
 #pragma unroll
@@ -323,11 +328,13 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, con
 #define DGV_MF_CASE(DD)                                                                               \
   case DD:                                                                                            \
     if (d_g)                                                                                          \
-      hipLaunchKernelGGL((k_pb_mfma<DD, true>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,    \
+      hipLaunchKernelGGL((k_pb_mfma<DD, true>), dim3(nitems), dim3(64 * kPbWaves), 0, ctx->stream,      \
+                         P.d_items,                                                                   \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_g,           \
                          P.d_partial, g.Mv, g.Mw);                                                    \
     else                                                                                              \
-      hipLaunchKernelGGL((k_pb_mfma<DD, false>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,   \
+      hipLaunchKernelGGL((k_pb_mfma<DD, false>), dim3(nitems), dim3(64 * kPbWaves), 0, ctx->stream,     \
+                         P.d_items,                                                                   \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_f,           \
                          P.d_partial, g.Mv, g.Mw);                                                    \
     break;

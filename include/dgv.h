@@ -194,6 +194,14 @@ int dgv_batch_eval_q(dgv_batch* b);
 int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_t nsteps,
                      int use_graph);
 int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms);
+/* segment size (pairs; 0 = default) and persistent-grid size (waves per CU;
+ * 0 = keep) of the batched evaluation */
+int dgv_batch_set_tile(dgv_batch* b, int32_t pairs_per_segment, int32_t waves_per_cu);
+/* on != 0: every evaluation of the batch (dgv_batch_eval_q and the RK stages
+ * of dgv_batch_rk_run) projects each instance's Q in FP64 onto the complement
+ * of the collision invariants -- the conservation-moment correction.  Off by
+ * default (the reference applies none). */
+int dgv_batch_set_projection(dgv_batch* b, int32_t on);
 /* device pointers of the packed f and q buffers */
 int dgv_batch_device_buffers(dgv_batch* b, double** d_f, double** d_q);
 /* FP64 projection of every instance's q (packed buffer) onto the complement of

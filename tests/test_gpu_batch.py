@@ -111,3 +111,24 @@ def test_batch_probe_cells_and_projection(grids):
         assert np.all(np.abs(mom) <= 1e-13 * scale)
         assert rel_l2(q, G.project_invariants(raw[i])) <= 1e-13
     b.close()
+
+
+def test_batch_projection_in_rk_conserves_moments(grids):
+    import dgv0d3v_direct_collision_amd as dgv
+    rng = np.random.default_rng(21)
+    which = np.array([0, 1, 1, 0, 1], np.int32)
+    b = dgv.DGVBatch(grids, which, dgv.DGV_FP32_COMP)
+    b.set_projection(True)
+    fs = [vdf(grids[g], rng, 2) for g in which]
+    for i, f in enumerate(fs):
+        b.set_f(i, f)
+    b.rk_run(0.02, 3, rkmts=3, use_graph=True)
+    for i, g in enumerate(which):
+        G = grids[g]
+        psi = np.stack([np.ones(G.nnodes), G.nodes_u, G.nodes_v, G.nodes_w,
+                        G.nodes_u ** 2 + G.nodes_v ** 2 + G.nodes_w ** 2])
+        m0 = (fs[i] * G.nodes_gwts) @ psi.T
+        m1 = (b.get_f(i) * G.nodes_gwts) @ psi.T
+        scale = np.abs(fs[i] * G.nodes_gwts) @ np.abs(psi.T)
+        assert np.all(np.abs(m1 - m0) <= 1e-13 * scale), (i, m1 - m0)
+    b.close()
