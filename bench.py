@@ -82,7 +82,8 @@ def main():
     ap.add_argument("--box", type=float, default=4.0)
     ap.add_argument("--nquad", type=int, default=24)
     ap.add_argument("--rk-steps", type=int, default=500)
-    ap.add_argument("--cpu-nodes", type=int, default=32)
+    ap.add_argument("--cpu-nodes", type=int, default=1536, help="nodes in the CPU baseline's sample")
+    ap.add_argument("--tile", type=int, default=0, help="batched segment size (0: autotune)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -160,10 +161,11 @@ def main():
         b = dgv.DGVBatch([d], which, getattr(dgv, "DGV_" + prec))
         for i in range(B):
             b.set_f(i, fs[i])
-        if prec == "FP32_COMP":
+        if prec == "FP32_COMP" and args.tile <= 0:
             tb, tms = b.autotune()
             log(f"batch {prec}: autotune {tb} pairs/segment ({tms:.1f} ms)")
-        else:   # the FP32 path's segment size
+        else:   # the given size, or the FP32 path's
+            tb = args.tile if prec == "FP32_COMP" else tb
             b.set_tile(tb)
         for _ in range(args.warmup):
             b.eval_q()
@@ -307,14 +309,18 @@ def main():
 
 
 def copy_probe():
-    """A copy kernel of known size: 2^28 float32 (1 GiB read, 1 GiB written) per
-    launch, for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
-    import torch
-    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
-    y = torch.empty_like(x)
+    """The library's plain copy kernel (k_copy16) on 1 GiB, five launches: 1 GiB
+    read and 1 GiB written per launch, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import dgv0d3v_direct_collision_amd as dgv
+    d = dgv.DGV(1, 1, (-1.0, 1.0))
+    n = 1 << 30
+    a, b = d.device_alloc(n), d.device_alloc(n)
     for _ in range(5):
-        y.copy_(x)
-    torch.cuda.synchronize()
+        d.device_copy(b, a, n)
+    d.device_free(a)
+    d.device_free(b)
+    d.close()
     log("copy probe: 5 copies of 1 GiB")
 
 

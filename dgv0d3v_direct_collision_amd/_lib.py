@@ -121,6 +121,7 @@ SIGNATURES = {
     "dgv_device_free": (C.c_int, [_vp, _vp]),
     "dgv_memcpy_htod": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "dgv_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "dgv_device_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
 }
 
 _lib = None

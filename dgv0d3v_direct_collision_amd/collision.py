@@ -348,6 +348,9 @@ class DGV:
         self._chk(self._L.dgv_memcpy_dtoh(self._ctx, a.ctypes.data_as(C.c_void_p), C.c_void_p(int(src)),
                                           a.nbytes))
 
+    def device_copy(self, dst: int, src: int, nbytes: int):
+        self._chk(self._L.dgv_device_copy(self._ctx, C.c_void_p(int(dst)), C.c_void_p(int(src)), nbytes))
+
     def eval_q_dev(self, d_f: int, d_q: int):
         self._chk(self._L.dgv_eval_q_dev(self._ctx, C.c_void_p(int(d_f)), C.c_void_p(int(d_q)), self.nnodes))
 

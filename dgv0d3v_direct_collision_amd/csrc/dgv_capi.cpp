@@ -766,4 +766,13 @@ int dgv_memcpy_dtoh(dgv_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return DGV_OK;
 }
 
+int dgv_device_copy(dgv_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  CHECK_CTX(ctx);
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  int rc = launch_copy16(ctx, dst, src, bytes);
+  if (rc) return rc;
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
 }  // extern "C"

@@ -165,6 +165,7 @@ int launch_eval_pairblock(dgv_ctx* ctx, const double* d_f, double* d_q, double s
 int launch_eval_record(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale);
 int eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* d_q);
+int launch_copy16(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int launch_axpy(dgv_ctx* ctx, int64_t n, double* out, const double* x0, int nterms,
                 const double* const* xs, const double* coefs);
 int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);

@@ -505,6 +505,24 @@ struct LinComb {
   int n;
 };
 
+// Plain device copy of 16-byte words (the byte counters of a profiler are
+// calibrated against it: n16 * 16 bytes read and written).
+__global__ void k_copy16(int64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int launch_copy16(dgv_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes % 16 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16)
+    return set_error(ctx, DGV_ERR_ARG, "device copy needs 16-byte aligned sizes and pointers");
+  const int64_t n16 = (int64_t)(bytes / 16);
+  if (n16 > 0)
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, ctx->stream, n16, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst));
+  DGV_HIP(ctx, hipGetLastError());
+  return DGV_OK;
+}
+
 __global__ void k_lincomb(int64_t n, double* __restrict__ out, LinComb c) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

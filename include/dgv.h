@@ -265,6 +265,8 @@ int dgv_device_alloc(dgv_ctx* ctx, size_t bytes, void** dptr);
 int dgv_device_free(dgv_ctx* ctx, void* dptr);
 int dgv_memcpy_htod(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
 int dgv_memcpy_dtoh(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* device-to-device copy by a plain kernel (16-byte aligned pointers and size) */
+int dgv_device_copy(dgv_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }
