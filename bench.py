@@ -200,7 +200,22 @@ def main():
         e = [rel_l2(qb["FP32_COMP"][i], qb["FP64"][i]) for i in range(B)]
         emax = [float(np.max(np.abs(qb["FP32_COMP"][i] - qb["FP64"][i])) / np.max(np.abs(qb["FP64"][i])))
                 for i in range(B)]
-        err32 = {"rel_l2_max": max(e), "rel_l2_mean": float(np.mean(e)), "rel_max_norm_max": max(emax)}
+        err32 = {"rel_l2_max": max(e), "rel_l2_mean": float(np.mean(e)), "rel_l2_median": float(np.median(e)),
+                 "rel_max_norm_max": max(emax), "frac_vdfs_within_1e-6": float(np.mean(np.array(e) <= 1e-6))}
+        # conditioning of the sum: Qabs = 2/w sum |A| f f (same evaluator on |A|)
+        dabs = dgv.DGV(M, 3, (-L, L))
+        dabs.generate_aarray(min_sens=0.0, nquad=args.nquad)
+        dabs.aarray_abs()
+        kap, sc = [], []
+        for i in range(B):
+            qa = dabs.EvalCollisionPeriodicAPlus_DGV(fs[i])
+            kap.append(float(np.linalg.norm(qa) / np.linalg.norm(qb["FP64"][i])))
+            sc.append(float(np.linalg.norm(qb["FP32_COMP"][i] - qb["FP64"][i]) / np.linalg.norm(qa)))
+        dabs.close()
+        err32.update({"kappa_median": float(np.median(kap)), "kappa_max": max(kap),
+                      "err_over_term_scale_max": max(sc), "err_over_term_scale_in_2^-24": max(sc) * 2.0 ** 24})
+        log(f"conditioning ||sum|A f f| || / ||Q||: median {np.median(kap):.1f}, max {max(kap):.1f}; "
+            f"FP32-comp error / term scale max {max(sc):.2e} ({max(sc) * 2 ** 24:.2f} x 2^-24)")
         results["FP64"]["rel_l2_vs_single_vdf"] = rel_l2(qb["FP64"][0], q_single)
         log(f"FP32-comp vs FP64: rel L2 max {max(e):.3e}; FP64 batch vs single {results['FP64']['rel_l2_vs_single_vdf']:.2e}")
     else:

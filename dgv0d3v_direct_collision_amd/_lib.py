@@ -68,6 +68,7 @@ SIGNATURES = {
     "dgv_ctx_read_aarray_file": (C.c_int, [_vp, C.c_char_p]),
     "dgv_ctx_trunc_aarray_threshold": (C.c_int, [_vp, _dbl]),
     "dgv_ctx_trunc_aarray_radius": (C.c_int, [_vp, _dbl]),
+    "dgv_ctx_aarray_abs": (C.c_int, [_vp]),
     "dgv_aarray_chunk_name": (C.c_int, [C.POINTER(GridDesc), C.c_char_p, C.c_char_p, _i32, _i32,
                                         _i32, C.c_char_p, C.c_size_t]),
     "dgv_eval_q": (C.c_int, [_vp, _dp, _dp, _i64]),

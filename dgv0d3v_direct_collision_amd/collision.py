@@ -226,6 +226,11 @@ class DGV:
         (dgvtools_mod.f90:1375-1429)."""
         self._chk(self._L.dgv_ctx_trunc_aarray_radius(self._ctx, rad))
 
+    def aarray_abs(self):
+        """Replaces every coefficient by |A| (diagnostic: Q then becomes the
+        scale sum 2/w |A| f f of the terms of Q)."""
+        self._chk(self._L.dgv_ctx_aarray_abs(self._ctx))
+
     def set_kernel(self, kernel: int):
         self._chk(self._L.dgv_ctx_set_kernel(self._ctx, kernel))
 

@@ -170,11 +170,37 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   return build_plan(ctx, ctx->tile);
 }
 
+__global__ void k_abs_coeff(int64_t npairs, int D, double* __restrict__ pairs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs * (D + 1)) return;
+  if (i % (D + 1) < D) pairs[i] = fabs(pairs[i]);
+}
+
 }  // namespace dgv
 
 using namespace dgv;
 
 extern "C" {
+
+int dgv_ctx_aarray_abs(dgv_ctx* ctx) {
+  if (!ctx) { set_global_error("null context"); return DGV_ERR_ARG; }
+  if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array");
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t tot = ctx->pb.npairs * (ctx->g.D + 1);
+  if (tot > 0)
+    hipLaunchKernelGGL(k_abs_coeff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, ctx->pb.npairs,
+                       ctx->g.D, ctx->pb.d_pairs);
+  DGV_HIP(ctx, hipGetLastError());
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  hipFree(ctx->d_rec_off); hipFree(ctx->d_rec_A); hipFree(ctx->d_rec_xi); hipFree(ctx->d_rec_xi1);
+  ctx->d_rec_off = nullptr; ctx->d_rec_A = nullptr; ctx->d_rec_xi = nullptr; ctx->d_rec_xi1 = nullptr;
+  ctx->has_rec = false;
+  ctx->exp_valid = false;
+  ctx->exp_idx.clear();
+  ctx->exp_val.clear();
+  return DGV_OK;
+}
+
 
 int dgv_ctx_trunc_aarray_threshold(dgv_ctx* ctx, double trhld) {
   if (!ctx) { set_global_error("null context"); return DGV_ERR_ARG; }

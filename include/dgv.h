@@ -119,6 +119,10 @@ int dgv_ctx_read_aarray_file(dgv_ctx* ctx, const char* path);
  * device copy in place and keep the records' order; batches and surrogates
  * built on the context before the call must be created again. */
 int dgv_ctx_trunc_aarray_threshold(dgv_ctx* ctx, double trhld);
+/* Diagnostic, no counterpart in the reference: replaces every coefficient by its
+ * absolute value.  For f >= 0 the evaluator then returns 2/w sum |A| f f, the
+ * scale of the terms whose sum is Q (the conditioning of the sum). */
+int dgv_ctx_aarray_abs(dgv_ctx* ctx);
 int dgv_ctx_trunc_aarray_radius(dgv_ctx* ctx, double rad);
 /* reference chunk-file base name (MakeBaseNameAoperChnksDGV, readwrite.f90:2224-2281) */
 int dgv_aarray_chunk_name(const dgv_grid_desc* grid, const char* dir, const char* base,
