@@ -90,6 +90,7 @@ struct dgv_batch {
   double* d_tmp[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int32_t*> d_blist;
   std::vector<std::vector<int64_t>> cell_ptr;  // per grid: ncells+1 offsets into blist
+  std::vector<std::vector<int32_t>> h_blist;   // per grid: host copy of the per-cell block lists
   std::vector<float*> d_pairs32;
   dgv::BatchGridDev* d_grids = nullptr;
   int32_t waves_per_cu = 0;
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   constexpr int CH = chunks_per_item<FP32>();    // chunks per item
   constexpr int WPC = kWavesPerItem / CH;        // waves per chunk
   constexpr int CT = 4 / WPC;                    // column tiles per wave
-  constexpr int kAhead = 4;                      // steps of coefficient loads in flight
+  constexpr int kAhead = FP32 ? 8 : 4;           // steps of coefficient loads in flight
   using T = typename std::conditional<FP32, float, double>::type;
   using V4 = typename std::conditional<FP32, bf4, bd4>::type;
   __shared__ T lfa[CH][D * 64], lfb[CH][D * 64];
@@ -207,21 +208,27 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       }
     T* la = lfa[ch];
     T* lb = lfb[ch];
+    int prev_xc = -1;   // the cell whose f is in la (blocks come ordered by a)
     for (int bi = it.b0; bi < it.b1; ++bi) {
       const BlockHdr* B = G.blocks + G.blist[bi];
       const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
       const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+      const bool new_x = xc != prev_xc;
+      prev_xc = xc;
       __syncthreads();
       if (active) {
         if constexpr (FP32) {
           const float* F = f32 + G.foff;
+          if (new_x) {
 #pragma unroll
-          for (int p = 0; p < D; ++p) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+            for (int p = 0; p < D; ++p) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+          }
 #pragma unroll
           for (int q = 0; q < D; ++q) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
         } else {
           const double* F = f64 + G.foff;
-          for (int p = sub; p < D; p += WPC) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+          if (new_x)
+            for (int p = sub; p < D; p += WPC) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
           for (int q = sub; q < D; q += WPC) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
         }
       }
@@ -462,20 +469,7 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
     const dgv_ctx* ctx = b->grids[gi];
     const PairBlock& pb = ctx->pb;
     const std::vector<int64_t>& cp = b->cell_ptr[gi];
-    std::vector<int32_t> blist(cp.back());
-    {
-      // the same per-cell block order as in the device list
-      std::vector<int64_t> pos(cp.begin(), cp.end() - 1);
-      for (const Group& G : pb.groups)
-        for (int r = 0; r < G.ncells; ++r) {
-          int rr = r;
-          const int cw = G.lo[2] + rr % G.ext[2]; rr /= G.ext[2];
-          const int cv = G.lo[1] + rr % G.ext[1]; rr /= G.ext[1];
-          const int cu = G.lo[0] + rr;
-          const int64_t cell = ((int64_t)cu * ctx->g.Mv + cv) * ctx->g.Mw + cw;
-          for (int k = G.bbeg; k < G.bend; ++k) blist[pos[cell]++] = pb.group_blocks[k];
-        }
-    }
+    const std::vector<int32_t>& blist = b->h_blist[gi];
     const int nchunk = b->Bpad[gi] / 64;
     std::vector<int64_t> todo;
     if (cells) {
@@ -700,6 +694,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
   b->d_blist.assign(ngrids, nullptr);
   b->d_pairs32.assign(ngrids, nullptr);
   b->cell_ptr.assign(ngrids, {});
+  b->h_blist.assign(ngrids, {});
   for (int g = 0; g < ngrids; ++g) {
     const dgv_ctx* ctx = grids[g];
     const PairBlock& pb = ctx->pb;
@@ -724,10 +719,19 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
         const int64_t cell = ((int64_t)cu * ctx->g.Mv + cv) * ctx->g.Mw + cw;
         for (int k = G.bbeg; k < G.bend; ++k) blist[pos[cell]++] = pb.group_blocks[k];
       }
+    // each cell's blocks ordered by (a, b): runs of equal a share the staged f[c + a]
+    for (int64_t c = 0; c < ctx->g.ncells; ++c)
+      std::sort(blist.begin() + cnt[c], blist.begin() + cnt[c + 1], [&](int32_t x, int32_t y) {
+        const BlockHdr &X = pb.blocks[x], &Y = pb.blocks[y];
+        for (int d = 0; d < 3; ++d) if (X.a[d] != Y.a[d]) return X.a[d] < Y.a[d];
+        for (int d = 0; d < 3; ++d) if (X.b[d] != Y.b[d]) return X.b[d] < Y.b[d];
+        return x < y;
+      });
     b->cell_ptr[g] = cnt;
     if (hipMalloc(&b->d_blist[g], sizeof(int32_t) * std::max<size_t>(blist.size(), 1)) ||
         hipMemcpy(b->d_blist[g], blist.data(), sizeof(int32_t) * blist.size(), hipMemcpyHostToDevice))
       return fail(DGV_ERR_NOMEM, "block lists");
+    b->h_blist[g] = blist;
     if (precision == DGV_FP32_COMP) {
       const int64_t n = pb.npairs * (D + 1);
       if (hipMalloc(&b->d_pairs32[g], sizeof(float) * std::max<int64_t>(n, 1)))

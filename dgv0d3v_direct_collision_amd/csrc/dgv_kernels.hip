@@ -162,16 +162,22 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = dgv_d4{0.0, 0.0, 0.0, 0.0};
 
+  int prev_a = -1;   // offset a whose f is staged in lfa (a group's blocks come ordered by a)
   for (int bi = it.bbeg; bi < it.bend; ++bi) {
     const BlockHdr* B = blocks + gblocks[bi];
     const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
     const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+    const int akey = ((B->a[0] + 64) << 16) | ((B->a[1] + 64) << 8) | (B->a[2] + 64);
+    const bool new_a = akey != prev_a;
+    prev_a = akey;
     __syncthreads();  // previous block's LDS reads are done
     // thread (wave, lane) stages the nodes p = wave, wave + 2, ... of its lane's cell
-    for (int p = wave; p < D; p += kPbWaves) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
+    if (new_a)
+      for (int p = wave; p < D; p += kPbWaves) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
     for (int q = wave; q < D; q += kPbWaves) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
     if constexpr (MIX) {
-      for (int p = wave; p < D; p += kPbWaves) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
+      if (new_a)
+        for (int p = wave; p < D; p += kPbWaves) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
       for (int q = wave; q < D; q += kPbWaves) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
     }
     __syncthreads();
