@@ -125,6 +125,7 @@ static int berr(dgv_batch* b, int code, const std::string& m) {
   } while (0)
 
 typedef double bd4 __attribute__((ext_vector_type(4)));
+#define AS1 __attribute__((address_space(1)))   // global memory: plain global loads, not flat
 typedef float bf4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -174,7 +175,6 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   constexpr int CH = chunks_per_item<FP32>();    // chunks per item
   constexpr int WPC = kWavesPerItem / CH;        // waves per chunk
   constexpr int CT = 4 / WPC;                    // column tiles per wave
-  constexpr int kAhead = FP32 ? 8 : 4;           // steps of coefficient loads in flight
   using T = typename std::conditional<FP32, float, double>::type;
   using V4 = typename std::conditional<FP32, bf4, bd4>::type;
   __shared__ T lfa[CH][D * 64], lfb[CH][D * 64];
@@ -233,68 +233,83 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         }
       }
       __syncthreads();
-      if (!active) continue;
       const int n = B->npairs;
-      const T* P;
-      if constexpr (FP32) P = G.pairs32 + B->pbeg * (D + 1);
-      else P = G.pairs64 + B->pbeg * (D + 1);
-      // coefficient operand of step s: lane i holds T[pair s + i/16][16 rt + i%16]
-      T ab[kAhead + 1][RT];
-      int cb[kAhead + 1];
+      if (!active || n == 0) continue;
+      // Coefficient operand of step s (4 pairs): lane i holds T[pair s + i/16][16 rt + i%16].
+      // A ring of kRing steps is kept in flight; the loads are unconditional
+      // global loads (indices clamped into the block's records; rows >= D read
+      // the record's code slot and only feed output rows that are dropped), and
+      // steps past the block's end carry code -1, so their products are 0.
+      constexpr int kRing = 8;
+      const AS1 T* P;
+      if constexpr (FP32) P = (const AS1 T*)(G.pairs32 + B->pbeg * (D + 1));
+      else P = (const AS1 T*)(G.pairs64 + B->pbeg * (D + 1));
+      T ab[kRing][RT];
+      int cbits[kRing];
+      bool cok[kRing];
 #define BATCH_LOAD(slot, s)                                                                   \
   do {                                                                                        \
-    const int pr_ = (s) + kk;                                                                 \
-    const bool ok_ = pr_ < n;                                                                 \
-    const T* R_ = P + (int64_t)pr_ * (D + 1);                                                 \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
-      ab[slot][rt] = (ok_ && 16 * rt + jj < D) ? R_[16 * rt + jj] : (T)0;                     \
-    if constexpr (FP32) cb[slot] = ok_ ? __float_as_int(R_[D]) : -1;                          \
-    else cb[slot] = ok_ ? (int)__double_as_longlong(R_[D]) : -1;                              \
+    const int pv_ = (s) + kk;                                                                 \
+    const AS1 T* R_ = P + (int64_t)min(pv_, n - 1) * (D + 1);                                 \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = R_[min(16 * rt + jj, D)]; \
+    if constexpr (FP32) cbits[slot] = __float_as_int(R_[D]);                                  \
+    else cbits[slot] = (int)__double_as_longlong(R_[D]);                                      \
+    cok[slot] = pv_ < n;                                                                      \
+  } while (0)
+      // the B operand of a step: LDS reads issued one step ahead of its products
+      T ra[CT], rb[CT], bcur[CT];
+      bool bok;
+#define B_ISSUE(slot)                                                                         \
+  do {                                                                                        \
+    bok = cok[slot];                                                                          \
+    const int cz_ = bok ? cbits[slot] : 0;                                                    \
+    const int p_ = cz_ >> 8, q_ = cz_ & 255;                                                  \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                       \
+      ra[ct] = la[p_ * 64 + col0 + 16 * ct + jj];                                             \
+      rb[ct] = lb[q_ * 64 + col0 + 16 * ct + jj];                                             \
+    }                                                                                         \
+  } while (0)
+#define B_FINISH()                                                                            \
+  do {                                                                                        \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = bok ? ra[ct] * rb[ct] : (T)0; \
   } while (0)
 #pragma unroll
-      for (int k = 0; k < kAhead; ++k) BATCH_LOAD(k, 4 * k);
-      constexpr int kFold = 8;   // FP32: fold every 8 steps (32 pairs)
-      int nstep = 0;
-      for (int s = 0; s < n; s += 4) {
-        BATCH_LOAD(kAhead, s + 4 * kAhead);
-        const int code = cb[0];
-        const int p = code >> 8, q = code & 255;
+      for (int u = 0; u < kRing; ++u) BATCH_LOAD(u, 4 * u);
+      B_ISSUE(0);
+      B_FINISH();
+      for (int s0 = 0; s0 < n; s0 += 4 * kRing) {
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          const int v = col0 + 16 * ct + jj;
-          const T b = code >= 0 ? la[p * 64 + v] * lb[q * 64 + v] : (T)0;
+        for (int u = 0; u < kRing; ++u) {
+          B_ISSUE((u + 1) % kRing);   // next step (slot 0 already holds the next round's first)
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt) {
-            if constexpr (FP32)
-              acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
-            else
-              acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
-          }
+          for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+              if constexpr (FP32)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
+              else
+                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
+            }
+          BATCH_LOAD(u, s0 + 4 * u + 4 * kRing);
+          B_FINISH();
         }
+        if constexpr (FP32) {  // fold the sums of these 32 pairs into (hi, lo)
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) {
-          cb[k] = cb[k + 1];
+          for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt) ab[k][rt] = ab[k + 1][rt];
-        }
-        if constexpr (FP32) {
-          if (++nstep == kFold || s + 4 >= n) {
-            nstep = 0;
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-              for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                  float s2, er;
-                  two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
-                  hi[rt][ct][e] = s2;
-                  lo[rt][ct][e] += er;
-                  acc[rt][ct][e] = 0.0f;
-                }
-          }
+              for (int e = 0; e < 4; ++e) {
+                float s2, er;
+                two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
+                hi[rt][ct][e] = s2;
+                lo[rt][ct][e] += er;
+                acc[rt][ct][e] = 0.0f;
+              }
         }
       }
+#undef B_FINISH
+#undef B_ISSUE
 #undef BATCH_LOAD
       if constexpr (!FP32) {  // two-level FP64 sum
 #pragma unroll

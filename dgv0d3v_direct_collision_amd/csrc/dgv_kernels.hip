@@ -125,6 +125,7 @@ __global__ __launch_bounds__(64) void k_pb_eval(const WorkItem* __restrict__ ite
 // the instruction: A[i][k] and B[k][j] in lane i|j + 16 k; D[i][j] in lane
 // j + 16 (i%4), element i/4 (the FP64 output layout).
 typedef double dgv_d4 __attribute__((ext_vector_type(4)));
+#define AS1 __attribute__((address_space(1)))   // global memory: plain global loads, not flat
 
 constexpr int kPbWaves = 2;
 
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
                                                            double* __restrict__ partial, int Mv, int Mw) {
   constexpr int RT = (D + 15) / 16;
   constexpr int CT = 4 / kPbWaves;
-  constexpr int kAhead = 4;
+  constexpr int kRing = 8;
   __shared__ double lfa[D * 64], lfb[D * 64];
   __shared__ double lga[MIX ? D * 64 : 1], lgb[MIX ? D * 64 : 1];
   const WorkItem it = items[blockIdx.x];
@@ -181,47 +182,70 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
       for (int q = wave; q < D; q += kPbWaves) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
     }
     __syncthreads();
-    const double* P = pairs + B->pbeg * (D + 1);
     const int n = B->npairs;
-    double ab[kAhead + 1][RT];
-    int cb[kAhead + 1];
+    if (n == 0) continue;
+    // Ring of kRing steps of coefficient loads in flight: unconditional global
+    // loads (indices clamped into the block's records; rows >= D read the code
+    // slot and only feed dropped output rows), steps past the end carry an
+    // invalid code and contribute 0.  The LDS reads of the next step's B
+    // operand are issued before the current step's products.
+    const AS1 double* P = (const AS1 double*)(pairs + B->pbeg * (D + 1));
+    double ab[kRing][RT];
+    int cbits[kRing];
+    bool cok[kRing];
 #define PB_LOAD(slot, s)                                                                  \
   do {                                                                                    \
-    const int pr_ = (s) + kk;                                                             \
-    const bool ok_ = pr_ < n;                                                             \
-    const double* R_ = P + (int64_t)pr_ * (D + 1);                                        \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
-      ab[slot][rt] = (ok_ && 16 * rt + jj < D) ? R_[16 * rt + jj] : 0.0;                  \
-    cb[slot] = ok_ ? (int)__double_as_longlong(R_[D]) : -1;                               \
+    const int pv_ = (s) + kk;                                                             \
+    const AS1 double* R_ = P + (int64_t)min(pv_, n - 1) * (D + 1);                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = R_[min(16 * rt + jj, D)]; \
+    cbits[slot] = (int)__double_as_longlong(R_[D]);                                       \
+    cok[slot] = pv_ < n;                                                                  \
+  } while (0)
+    double xa[CT], xb[CT], ya[MIX ? CT : 1], yb[MIX ? CT : 1], bcur[CT];
+    bool bok;
+#define PB_ISSUE(slot)                                                                    \
+  do {                                                                                    \
+    bok = cok[slot];                                                                      \
+    const int cz_ = bok ? cbits[slot] : 0;                                                \
+    const int p_ = cz_ >> 8, q_ = cz_ & 255;                                              \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                   \
+      const int cell_ = col0 + 16 * ct + jj;                                              \
+      xa[ct] = lfa[p_ * 64 + cell_];                                                      \
+      xb[ct] = lfb[q_ * 64 + cell_];                                                      \
+      if constexpr (MIX) {                                                                \
+        ya[ct] = lga[p_ * 64 + cell_];                                                    \
+        yb[ct] = lgb[q_ * 64 + cell_];                                                    \
+      }                                                                                   \
+    }                                                                                     \
+  } while (0)
+#define PB_FINISH()                                                                       \
+  do {                                                                                    \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                   \
+      double prod_;                                                                       \
+      if constexpr (MIX) prod_ = __dadd_rn(__dmul_rn(ya[ct], xb[ct]), __dmul_rn(xa[ct], yb[ct])); \
+      else prod_ = xa[ct] * xb[ct];                                                       \
+      bcur[ct] = bok ? prod_ : 0.0;                                                       \
+    }                                                                                     \
   } while (0)
 #pragma unroll
-    for (int k = 0; k < kAhead; ++k) PB_LOAD(k, 4 * k);
-    for (int s = 0; s < n; s += 4) {
-      PB_LOAD(kAhead, s + 4 * kAhead);
-      const int c0 = cb[0];
-      const int p = c0 >> 8, q = c0 & 255;
+    for (int u = 0; u < kRing; ++u) PB_LOAD(u, 4 * u);
+    PB_ISSUE(0);
+    PB_FINISH();
+    for (int s0 = 0; s0 < n; s0 += 4 * kRing) {
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int cell = col0 + 16 * ct + jj;
-        double b = 0.0;
-        if (c0 >= 0) {
-          if constexpr (MIX)
-            b = __dadd_rn(__dmul_rn(lga[p * 64 + cell], lfb[q * 64 + cell]),
-                          __dmul_rn(lfa[p * 64 + cell], lgb[q * 64 + cell]));
-          else
-            b = lfa[p * 64 + cell] * lfb[q * 64 + cell];
-        }
+      for (int u = 0; u < kRing; ++u) {
+        PB_ISSUE((u + 1) % kRing);
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[0][rt], b, acc[rt][ct], 0, 0, 0);
-      }
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-      for (int k = 0; k < kAhead; ++k) {
-        cb[k] = cb[k + 1];
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) ab[k][rt] = ab[k + 1][rt];
+          for (int rt = 0; rt < RT; ++rt)
+            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
+        PB_LOAD(u, s0 + 4 * u + 4 * kRing);
+        PB_FINISH();
       }
     }
+#undef PB_FINISH
+#undef PB_ISSUE
 #undef PB_LOAD
   }
 #pragma unroll
