@@ -78,6 +78,7 @@ SIGNATURES = {
     "dgv_eval_mixed_dev": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),
     "dgv_ctx_set_kernel": (C.c_int, [_vp, C.c_int]),
     "dgv_ctx_set_tile": (C.c_int, [_vp, _i32]),
+    "dgv_ctx_set_projection": (C.c_int, [_vp, _i32]),
     "dgv_ctx_autotune": (C.c_int, [_vp, _ip, _dp]),
     "dgv_spatial_oper": (C.c_int, [_vp, _dp, _dp, _i64, _dbl, _dbl]),
     "dgv_spatial_oper_dev": (C.c_int, [_vp, _vp, _vp, _i64, _dbl, _dbl]),
@@ -94,6 +95,9 @@ SIGNATURES = {
     "dgv_batch_layout": (C.c_int, [_vp, _i32, _lp, _lp, _lp]),
     "dgv_batch_set_f": (C.c_int, [_vp, _i32, _dp, _i64]),
     "dgv_batch_get_q": (C.c_int, [_vp, _i32, _dp, _i64]),
+    "dgv_batch_set_f_rows_dev": (C.c_int, [_vp, _vp, _i64, _i32, _i32]),
+    "dgv_batch_get_q_rows_dev": (C.c_int, [_vp, _vp, _i64, _i32, _i32]),
+    "dgv_batch_get_f_rows_dev": (C.c_int, [_vp, _vp, _i64, _i32, _i32]),
     "dgv_batch_eval_q": (C.c_int, [_vp]),
     "dgv_batch_rk_run": (C.c_int, [_vp, _dbl, _dbl, _i32, _i64, C.c_int]),
     "dgv_batch_autotune": (C.c_int, [_vp, _ip, _dp]),

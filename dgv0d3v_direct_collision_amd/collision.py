@@ -234,6 +234,10 @@ class DGV:
     def set_kernel(self, kernel: int):
         self._chk(self._L.dgv_ctx_set_kernel(self._ctx, kernel))
 
+    def set_projection(self, on: bool):
+        """Project every Q onto the complement of the collision invariants (FP64)."""
+        self._chk(self._L.dgv_ctx_set_projection(self._ctx, int(on)))
+
     def set_tile(self, pairs_per_item: int):
         self._chk(self._L.dgv_ctx_set_tile(self._ctx, pairs_per_item))
 
@@ -427,6 +431,19 @@ class DGVBatch:
         allf = np.zeros(total.value)
         g.dtoh(allf, df.value)
         return allf[off.value::stride.value][:g.nnodes].copy()
+
+    def set_f_rows(self, d_rows: int, first: int, count: int, ld: Optional[int] = None):
+        """Instances first .. first+count-1 (one grid) from device rows."""
+        n = self.grids[self.grid_of_instance[first]].nnodes
+        self._chk(self._L.dgv_batch_set_f_rows_dev(self._b, C.c_void_p(int(d_rows)), ld or n, first, count))
+
+    def get_q_rows(self, d_rows: int, first: int, count: int, ld: Optional[int] = None):
+        n = self.grids[self.grid_of_instance[first]].nnodes
+        self._chk(self._L.dgv_batch_get_q_rows_dev(self._b, C.c_void_p(int(d_rows)), ld or n, first, count))
+
+    def get_f_rows(self, d_rows: int, first: int, count: int, ld: Optional[int] = None):
+        n = self.grids[self.grid_of_instance[first]].nnodes
+        self._chk(self._L.dgv_batch_get_f_rows_dev(self._b, C.c_void_p(int(d_rows)), ld or n, first, count))
 
     def eval_q(self):
         self._chk(self._L.dgv_batch_eval_q(self._b))

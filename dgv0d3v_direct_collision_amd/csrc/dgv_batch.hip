@@ -458,6 +458,40 @@ __global__ __launch_bounds__(1024) void k_batch_project(const BatchProjGrid* __r
   }
 }
 
+// rows [count][ld] <-> packed instances (node * stride + first + r), 64 x 64 tiles
+__global__ __launch_bounds__(256) void k_rows_pack(const double* __restrict__ rows, int64_t ld, int count, int64_t N,
+                                                   double* __restrict__ packed, int64_t stride, int to_packed) {
+  __shared__ double tile[64][65];
+  const int64_t n0 = (int64_t)blockIdx.x * 64;
+  const int r0 = blockIdx.y * 64;
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  if (to_packed) {
+    for (int i = 0; i < 16; ++i) {
+      const int r = r0 + rg + 4 * i;
+      const int64_t n = n0 + c;
+      tile[rg + 4 * i][c] = (r < count && n < N) ? rows[(int64_t)r * ld + n] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < 16; ++i) {
+      const int64_t n = n0 + rg + 4 * i;
+      const int r = r0 + c;
+      if (n < N && r < count) packed[n * stride + r] = tile[c][rg + 4 * i];
+    }
+  } else {
+    for (int i = 0; i < 16; ++i) {
+      const int64_t n = n0 + rg + 4 * i;
+      const int r = r0 + c;
+      tile[rg + 4 * i][c] = (n < N && r < count) ? packed[n * stride + r] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < 16; ++i) {
+      const int r = r0 + rg + 4 * i;
+      const int64_t n = n0 + c;
+      if (r < count && n < N) rows[(int64_t)r * ld + n] = tile[c][rg + 4 * i];
+    }
+  }
+}
+
 static void destroy_plan(dgv_batch* b, BatchPlan& P) {
   hipFree(P.d_items); hipFree(P.d_partial); hipFree(P.d_red_rows); hipFree(P.d_red_ptr); hipFree(P.d_red_key);
   P.d_items = nullptr; P.d_partial = nullptr; P.d_red_rows = nullptr; P.d_red_ptr = nullptr; P.d_red_key = nullptr;
@@ -844,6 +878,41 @@ int dgv_batch_set_f(dgv_batch* b, int32_t instance, const double* f, int64_t n) 
 
 int dgv_batch_get_q(dgv_batch* b, int32_t instance, double* q, int64_t n) {
   return batch_copy(b, instance, q, n, false, b ? b->d_q : nullptr);
+}
+
+// instances first .. first + count - 1 (all on one grid) from / to device rows
+static int batch_rows(dgv_batch* b, const double* d_rows, int64_t ld, int32_t first, int32_t count, bool to_packed,
+                      double* packed) {
+  if (!b || !d_rows || first < 0 || count < 1 || first + count > b->ninst) {
+    set_global_error("bad batch row range");
+    return DGV_ERR_ARG;
+  }
+  const int g = b->inst_grid[first];
+  for (int32_t i = first; i < first + count; ++i)
+    if (b->inst_grid[i] != g || b->inst_local[i] != b->inst_local[first] + (i - first))
+      return berr(b, DGV_ERR_ARG, "row copies need consecutive instances of one grid");
+  const int64_t N = b->grids[g]->g.N;
+  if (ld < N) return berr(b, DGV_ERR_ARG, "leading dimension below the node count");
+  BHIP(b, hipSetDevice(b->grids[0]->device));
+  double* base = packed + b->foff[g] + b->inst_local[first];
+  hipLaunchKernelGGL(k_rows_pack, dim3((unsigned)((N + 63) / 64), (unsigned)((count + 63) / 64)), dim3(256), 0,
+                     b->stream, const_cast<double*>(d_rows), ld, count, N, base, (int64_t)b->Bpad[g],
+                     to_packed ? 1 : 0);
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return DGV_OK;
+}
+
+int dgv_batch_set_f_rows_dev(dgv_batch* b, const double* d_rows, int64_t ld, int32_t first, int32_t count) {
+  return batch_rows(b, d_rows, ld, first, count, true, b ? b->d_f : nullptr);
+}
+
+int dgv_batch_get_q_rows_dev(dgv_batch* b, double* d_rows, int64_t ld, int32_t first, int32_t count) {
+  return batch_rows(b, d_rows, ld, first, count, false, b ? b->d_q : nullptr);
+}
+
+int dgv_batch_get_f_rows_dev(dgv_batch* b, double* d_rows, int64_t ld, int32_t first, int32_t count) {
+  return batch_rows(b, d_rows, ld, first, count, false, b ? b->d_f : nullptr);
 }
 
 int dgv_batch_eval_q(dgv_batch* b) {

@@ -301,6 +301,13 @@ int dgv_ctx_set_kernel(dgv_ctx* ctx, int kernel) {
   return DGV_OK;
 }
 
+int dgv_ctx_set_projection(dgv_ctx* ctx, int32_t on) {
+  CHECK_CTX(ctx);
+  ctx->project = on ? 1 : 0;
+  if (ctx->rk_graph) { hipGraphExecDestroy(ctx->rk_graph); ctx->rk_graph = nullptr; }
+  return DGV_OK;
+}
+
 int dgv_ctx_set_tile(dgv_ctx* ctx, int32_t pairs_per_item) {
   CHECK_CTX(ctx);
   if (pairs_per_item < 0) return set_error(ctx, DGV_ERR_ARG, "negative tile");

@@ -110,6 +110,7 @@ struct dgv_ctx {
   dgv::Plan plan;
   int32_t tile = 0;          // pairs per work item (0 = default)
   int kernel = DGV_KERNEL_AUTO;
+  int project = 0;           // project every Q onto the complement of the invariants
   // reference layout on the device for the record kernel (built lazily)
   bool has_rec = false;
   int64_t* d_rec_off = nullptr;   // D+1 offsets of the canonical basis functions

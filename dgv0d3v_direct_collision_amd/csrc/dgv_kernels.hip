@@ -472,8 +472,10 @@ int eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* 
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set (dgv_ctx_set_aarray / generate)");
   const int k = resolve_kernel(ctx);
-  if (k == DGV_KERNEL_RECORD) return launch_eval_record(ctx, d_f, d_q, scale);
-  return launch_eval_pairblock(ctx, d_f, d_q, scale, k == DGV_KERNEL_MFMA);
+  int rc = k == DGV_KERNEL_RECORD ? launch_eval_record(ctx, d_f, d_q, scale)
+                                  : launch_eval_pairblock(ctx, d_f, d_q, scale, k == DGV_KERNEL_MFMA);
+  if (rc == DGV_OK && ctx->project) rc = device_project(ctx, d_q);
+  return rc;
 }
 
 // Nonzero coefficients per block (records of the reference layout).

@@ -144,6 +144,11 @@ int dgv_eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, doub
 int dgv_ctx_set_kernel(dgv_ctx* ctx, int kernel);
 /* tunables of the pair-block kernel (pairs per work item); 0 = keep */
 int dgv_ctx_set_tile(dgv_ctx* ctx, int32_t pairs_per_item);
+/* on != 0: every Q of the context (evaluations, spatial operator, RK/AB stages,
+ * graph replays) is projected in FP64 onto the complement of the collision
+ * invariants -- the conservation-moment correction.  Off by default: the
+ * reference applies none. */
+int dgv_ctx_set_projection(dgv_ctx* ctx, int32_t on);
 int dgv_ctx_autotune(dgv_ctx* ctx, int32_t* best_pairs_per_item, double* best_ms);
 
 /* SpatialOperSH_DGV(f,fcol,dt), run_mode 0 (spat_oper_mod.f90:28-68):
@@ -192,6 +197,11 @@ int dgv_batch_layout(const dgv_batch* b, int32_t instance, int64_t* offset, int6
 /* pack / unpack one instance (host arrays in reference node order) */
 int dgv_batch_set_f(dgv_batch* b, int32_t instance, const double* f, int64_t n);
 int dgv_batch_get_q(dgv_batch* b, int32_t instance, double* q, int64_t n);
+/* the same for `count` consecutive instances of one grid at once, from / to
+ * device rows (row r = instance first + r, leading dimension ld doubles) */
+int dgv_batch_set_f_rows_dev(dgv_batch* b, const double* d_rows, int64_t ld, int32_t first, int32_t count);
+int dgv_batch_get_q_rows_dev(dgv_batch* b, double* d_rows, int64_t ld, int32_t first, int32_t count);
+int dgv_batch_get_f_rows_dev(dgv_batch* b, double* d_rows, int64_t ld, int32_t first, int32_t count);
 /* Q of every instance, one persistent launch over the (instance, cell) queue */
 int dgv_batch_eval_q(dgv_batch* b);
 /* one RK step (rkmts 1..3) of every instance, optionally hipGraph-replayed */

@@ -157,3 +157,27 @@ def test_batch_projection_in_rk_conserves_moments(grids):
         scale = np.abs(fs[i] * G.nodes_gwts) @ np.abs(psi.T)
         assert np.all(np.abs(m1 - m0) <= 1e-13 * scale), (i, m1 - m0)
     b.close()
+
+
+def test_batch_device_rows_match_per_instance_copies(grids):
+    import dgv0d3v_direct_collision_amd as dgv
+    rng = np.random.default_rng(33)
+    which = np.array([1] * 70, np.int32)
+    g = grids[1]
+    b = dgv.DGVBatch(grids, which, dgv.DGV_FP64)
+    F = np.stack([vdf(g, rng, 1 + i % 3) for i in range(70)])
+    d_rows = g.device_alloc(F.nbytes)
+    d_q = g.device_alloc(F.nbytes)
+    g.htod(d_rows, F)
+    b.set_f_rows(d_rows, 0, 70)
+    for i in (0, 37, 69):
+        assert np.array_equal(b.get_f(i), F[i])
+    b.eval_q()
+    b.get_q_rows(d_q, 0, 70)
+    Q = np.zeros_like(F)
+    g.dtoh(Q, d_q)
+    for i in (0, 1, 63, 64, 69):
+        assert np.array_equal(Q[i], b.get_q(i))
+    g.device_free(d_rows)
+    g.device_free(d_q)
+    b.close()
