@@ -459,7 +459,7 @@ __global__ __launch_bounds__(1024) void k_batch_project(const BatchProjGrid* __r
 }
 
 // rows [count][ld] <-> packed instances (node * stride + first + r), 64 x 64 tiles
-__global__ __launch_bounds__(256) void k_rows_pack(const double* __restrict__ rows, int64_t ld, int count, int64_t N,
+__global__ __launch_bounds__(256) void k_rows_pack(double* __restrict__ rows, int64_t ld, int count, int64_t N,
                                                    double* __restrict__ packed, int64_t stride, int to_packed) {
   __shared__ double tile[64][65];
   const int64_t n0 = (int64_t)blockIdx.x * 64;
