@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the mixed-grid batching and 33^3 precision configurations")
     ap.add_argument("--sur-rows", type=int, default=2048, help="(f, Q) pairs generated for the surrogate")
     ap.add_argument("--sur-steps", type=int, default=400)
     ap.add_argument("--profile", action="store_true", help="evaluator loops only (for rocprofv3)")
@@ -242,6 +244,12 @@ def main():
               "projected": rk3_run(d, d_f, f, args.rk_steps, project=True)}
         d.set_projection(False)
 
+    # ---- the batching and precision configurations ------------------------------------------
+    mixed = prec33 = None
+    if not args.no_configs:
+        mixed = mixed_grid_leg(dgv, d, rng, args)
+        prec33 = precision_leg(dgv, rng, args)
+
     # ---- learned surrogate -----------------------------------------------------------------
     sur = None
     if not args.no_surrogate:
@@ -311,6 +319,8 @@ def main():
                          "after_projection_rel_max": proj_rel},
         "rk3": rk,
         "surrogate": sur,
+        "mixed_grid_batch": mixed,
+        "precision_33cubed": prec33,
     }
     print(json.dumps(out), flush=True)
     d.device_free(d_rows)
@@ -318,6 +328,105 @@ def main():
     d.device_free(d_f)
     d.device_free(d_q)
     d.close()
+
+
+def mixed_grid_leg(dgv, d27, rng, args):
+    """512 VDFs of uneven size in one persistent launch: 128 on 15^3 (5 cells,
+    [-3.5,3.5]^3), 128 on 21^3 (7 cells, [-4,4]^3), 256 on 27^3 (the main grid);
+    seeded 1-3 Maxwellian mixtures; FP32-compensated and FP64 timed; a
+    hipGraph-captured RK2 step of the whole batch."""
+    log("mixed grids: A-arrays for 15^3 and 21^3")
+    g15 = dgv.DGV(5, 3, (-3.5, 3.5))
+    g21 = dgv.DGV(7, 3, (-4.0, 4.0))
+    g15.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    g21.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    grids = [g15, g21, d27]
+    which = np.array([0] * 128 + [1] * 128 + [2] * 256, np.int32)
+    rng.shuffle(which)
+    fs = [mixture_vdf(g.nodes_u, g.nodes_v, g.nodes_w, g.nodes_gwts, rng, kmax=3) for g in (grids[k] for k in which)]
+    out, qs = {}, {}
+    visits = sum(grids[k].workload()["visits"] for k in which)
+    for prec in ("FP32_COMP", "FP64"):
+        b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + prec))
+        for i, f in enumerate(fs):
+            b.set_f(i, f)
+        b.set_tile(args.tile if args.tile > 0 else 262144)
+        b.eval_q()
+        t = time.perf_counter()
+        b.eval_q()
+        el = time.perf_counter() - t
+        qs[prec] = [b.get_q(i) for i in range(len(which))]
+        b.rk_run(0.005, 1, rkmts=2, use_graph=True)       # capture
+        t = time.perf_counter()
+        b.rk_run(0.005, 2, rkmts=2, use_graph=True)
+        rk_el = (time.perf_counter() - t) / 2
+        out[prec] = {"seconds_per_batch": el, "vdf_q_per_s": len(which) / el,
+                     "tflops": 2.0 * visits / el / 1e12, "rk2_graph_step_seconds": rk_el}
+        log(f"mixed grids {prec}: {el:.2f} s per 512 VDFs = {len(which) / el:.1f} VDF-Q/s; RK2 step {rk_el:.2f} s")
+        b.close()
+    e = np.array([rel_l2(qs["FP32_COMP"][i], qs["FP64"][i]) for i in range(len(which))])
+    out["fp32comp_error_vs_fp64"] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
+                                     "frac_vdfs_within_1e-6": float(np.mean(e <= 1e-6))}
+    out["grids"] = {"15^3": 128, "21^3": 128, "27^3": 256}
+    g15.close()
+    g21.close()
+    return out
+
+
+def precision_leg(dgv, rng, args):
+    """64 near-equilibrium VDFs on 33^3 (11 cells on [-4.5,4.5]^3): a Maxwellian
+    plus a seeded perturbation of relative amplitude 1e-3, 1e-2 or 1e-1;
+    FP32-compensated against the FP64 path (FP64 is held to the oracle by the
+    parity tests); relative L2 and max-norm errors per VDF, the conditioning
+    of each Q, and the moments after the FP64 projection."""
+    M = 11
+    log("precision config: 33^3 A-array")
+    g = dgv.DGV(M, 3, (-4.5, 4.5))
+    g.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    u, v, w, gw = g.nodes_u, g.nodes_v, g.nodes_w, g.nodes_gwts
+    amps = [1e-3, 1e-2, 1e-1]
+    fs, amp_of = [], []
+    for i in range(64):
+        a = amps[i % 3]
+        f0 = maxwellian(u, v, w, 1.0, rng.uniform(-0.3, 0.3, 3), rng.uniform(0.6, 1.0))
+        pert = rng.uniform(-1, 1, len(u)) * f0 * a
+        f = f0 + pert
+        fs.append(f / np.sum(gw * f))
+        amp_of.append(a)
+    which = np.zeros(64, np.int32)
+    qs, times = {}, {}
+    for prec in ("FP32_COMP", "FP64"):
+        b = dgv.DGVBatch([g], which, getattr(dgv, "DGV_" + prec))
+        for i, f in enumerate(fs):
+            b.set_f(i, f)
+        b.set_tile(args.tile if args.tile > 0 else 262144)
+        b.eval_q()
+        t = time.perf_counter()
+        b.eval_q()
+        times[prec] = time.perf_counter() - t
+        qs[prec] = [b.get_q(i) for i in range(64)]
+        if prec == "FP32_COMP":
+            b.project_invariants()
+            psi = np.stack([np.ones(len(u)), u, v, w, u * u + v * v + w * w])
+            qp = [b.get_q(i) for i in range(64)]
+            proj = max(float(np.max(np.abs((q * gw) @ psi.T) / (np.abs(q * gw) @ np.abs(psi.T)))) for q in qp)
+        b.close()
+    g.aarray_abs()
+    per = {}
+    for a in amps:
+        idx = [i for i in range(64) if amp_of[i] == a]
+        e = [rel_l2(qs["FP32_COMP"][i], qs["FP64"][i]) for i in idx]
+        em = [float(np.max(np.abs(qs["FP32_COMP"][i] - qs["FP64"][i])) / np.max(np.abs(qs["FP64"][i]))) for i in idx]
+        kap = [float(np.linalg.norm(g.EvalCollisionPeriodicAPlus_DGV(fs[i])) / np.linalg.norm(qs["FP64"][i]))
+               for i in idx]
+        per[f"{a:g}"] = {"vdfs": len(idx), "rel_l2_max": max(e), "rel_l2_median": float(np.median(e)),
+                         "rel_max_norm_max": max(em), "kappa_median": float(np.median(kap))}
+        log(f"precision 33^3, amplitude {a:g}: FP32-comp rel L2 max {max(e):.2e}, median {np.median(e):.2e}, "
+            f"kappa median {np.median(kap):.0f}")
+    g.close()
+    return {"nodes": 35937, "vdfs": 64, "per_perturbation_amplitude": per,
+            "fp32comp_vdf_q_per_s": 64 / times["FP32_COMP"], "fp64_vdf_q_per_s": 64 / times["FP64"],
+            "projected_moment_rel_max": proj}
 
 
 def rk3_run(d, d_f, f, nsteps, project):

@@ -102,6 +102,7 @@ struct dgv_batch {
   int32_t nproj_task = 0;
   int project = 0;                        // project every evaluation's Q (FP64)
   int* d_counter = nullptr;
+  double* d_stage = nullptr;       // one instance's values (host copies)
   int ncu = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -738,6 +739,11 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
   if (precision == DGV_FP32_COMP && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
     return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
   if (hipMalloc(&b->d_counter, sizeof(int) * 4)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
+  {
+    int64_t nmax = 1;
+    for (int g = 0; g < ngrids; ++g) nmax = std::max<int64_t>(nmax, grids[g]->g.N);
+    if (hipMalloc(&b->d_stage, sizeof(double) * nmax)) return fail(DGV_ERR_NOMEM, "hipMalloc(staging)");
+  }
   // per grid: per-cell block lists, FP32 coefficients
   std::vector<BatchGridDev> gd(ngrids);
   b->d_blist.assign(ngrids, nullptr);
@@ -836,7 +842,7 @@ void dgv_batch_destroy(dgv_batch* b) {
   destroy_plan(b, b->plan);
   destroy_plan(b, b->probe);
   hipFree(b->d_f); hipFree(b->d_q); hipFree(b->d_f32); hipFree(b->d_counter); hipFree(b->d_grids);
-  hipFree(b->d_proj); hipFree(b->d_proj_task);
+  hipFree(b->d_proj); hipFree(b->d_proj_task); hipFree(b->d_stage);
   for (int i = 0; i < 4; ++i) hipFree(b->d_tmp[i]);
   for (auto p : b->d_blist) hipFree(p);
   for (auto p : b->d_pairs32) hipFree(p);
@@ -855,19 +861,25 @@ int dgv_batch_layout(const dgv_batch* b, int32_t instance, int64_t* offset, int6
   return DGV_OK;
 }
 
+// One instance from / to a host array: a contiguous copy through a staging
+// buffer and the row pack kernel (a 2-D copy of 8-byte rows would be slow).
 static int batch_copy(dgv_batch* b, int32_t instance, double* host, int64_t n, bool to_device, double* dev) {
   if (!b || instance < 0 || instance >= b->ninst || !host) { set_global_error("bad argument"); return DGV_ERR_ARG; }
   const int g = b->inst_grid[instance];
   const int64_t N = b->grids[g]->g.N;
   if (n != N) return berr(b, DGV_ERR_ARG, "array length differs from the instance's node count");
   BHIP(b, hipSetDevice(b->grids[0]->device));
-  // strided copy through a 2-D memcpy: N rows of one double, pitch Bpad
   double* base = dev + b->foff[g] + b->inst_local[instance];
-  const size_t pitch = sizeof(double) * b->Bpad[g];
-  if (to_device)
-    BHIP(b, hipMemcpy2DAsync(base, pitch, host, sizeof(double), sizeof(double), N, hipMemcpyHostToDevice, b->stream));
-  else
-    BHIP(b, hipMemcpy2DAsync(host, sizeof(double), base, pitch, sizeof(double), N, hipMemcpyDeviceToHost, b->stream));
+  const dim3 grid((unsigned)((N + 63) / 64), 1);
+  if (to_device) {
+    BHIP(b, hipMemcpyAsync(b->d_stage, host, sizeof(double) * N, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_rows_pack, grid, dim3(256), 0, b->stream, b->d_stage, N, 1, N, base, (int64_t)b->Bpad[g], 1);
+    BHIP(b, hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_rows_pack, grid, dim3(256), 0, b->stream, b->d_stage, N, 1, N, base, (int64_t)b->Bpad[g], 0);
+    BHIP(b, hipGetLastError());
+    BHIP(b, hipMemcpyAsync(host, b->d_stage, sizeof(double) * N, hipMemcpyDeviceToHost, b->stream));
+  }
   BHIP(b, hipStreamSynchronize(b->stream));
   return DGV_OK;
 }
