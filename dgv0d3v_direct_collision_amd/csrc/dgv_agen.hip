@@ -195,11 +195,13 @@ __global__ void k_segments(int64_t ncp, const int64_t* __restrict__ off, const u
 template <int D>
 __global__ void k_compact(int64_t npairs, const uint8_t* __restrict__ keep, const int64_t* __restrict__ dst,
                           const double* __restrict__ src, double* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= npairs || !keep[k]) return;
-  const double* a = src + k * (D + 1);
-  double* b = out + dst[k] * (D + 1);
-  for (int l = 0; l <= D; ++l) b[l] = a[l];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += step) {
+    if (!keep[k]) continue;
+    const double* a = src + k * (D + 1);
+    double* b = out + dst[k] * (D + 1);
+    for (int l = 0; l <= D; ++l) b[l] = a[l];
+  }
 }
 
 // per-basis record counts (integer atomics: exact and order independent)
@@ -291,7 +293,7 @@ static int generate_t(dgv_ctx* ctx, const AgenConsts& C, double min_sens) {
   AG_HIP(hipMemcpy(d_dst, dst.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
   AG_HIP(hipMalloc(&d_pairs, sizeof(double) * (D + 1) * std::max<int64_t>(nk, 1)));
   if (np > 0)
-    hipLaunchKernelGGL(k_compact<D>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, d_keep, d_dst, d_raw,
+    hipLaunchKernelGGL(k_compact<D>, dim3(stride_blocks(np, 256)), dim3(256), 0, st, np, d_keep, d_dst, d_raw,
                        d_pairs);
   AG_HIP(hipMalloc(&d_kcnt, sizeof(int32_t) * ncp));
   AG_HIP(hipMalloc(&d_seg, sizeof(uint16_t) * (D + 1) * ncp));

@@ -130,16 +130,17 @@ typedef double bd4 __attribute__((ext_vector_type(4)));
 typedef float bf4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) b[i] = (float)a[i];
 }
 
 __global__ void k_pairs_to_f32(int64_t npairs, int D, const double* __restrict__ a, float* __restrict__ b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npairs * (D + 1)) return;
-  const int l = (int)(i % (D + 1));
-  if (l < D) b[i] = (float)a[i];
-  else b[i] = __int_as_float((int)__double_as_longlong(a[i]));  // pair code
+  const int64_t n = npairs * (D + 1), step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const int l = (int)(i % (D + 1));
+    if (l < D) b[i] = (float)a[i];
+    else b[i] = __int_as_float((int)__double_as_longlong(a[i]));  // pair code
+  }
 }
 
 __global__ void k_reset_counter(int* c) {
@@ -386,17 +387,18 @@ struct BLinComb {
   int n;
 };
 __global__ void k_blincomb(int64_t n, double* __restrict__ out, BLinComb c) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < c.n; ++k) {
-    double v = c.t[k].x[i];
-    if (c.t[k].m1 != 1.0) v = v * c.t[k].m1;
-    if (c.t[k].den != 1.0) v = v / c.t[k].den;
-    if (k == 0) s = c.t[k].neg ? -v : v;
-    else s = c.t[k].neg ? s - v : s + v;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    double s = 0.0;
+    for (int k = 0; k < c.n; ++k) {
+      double v = c.t[k].x[i];
+      if (c.t[k].m1 != 1.0) v = v * c.t[k].m1;
+      if (c.t[k].den != 1.0) v = v / c.t[k].den;
+      if (k == 0) s = c.t[k].neg ? -v : v;
+      else s = c.t[k].neg ? s - v : s + v;
+    }
+    out[i] = s;
   }
-  out[i] = s;
 }
 
 // FP64 projection of every instance's Q onto the complement of the collision
@@ -587,7 +589,7 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
   const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
   if (b->precision == DGV_FP32_COMP)
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
+    hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(b->total, 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
   // workgroups of four waves; waves_per_cu sets the persistent grid's size
   const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
@@ -632,7 +634,7 @@ static int blincomb(dgv_batch* b, double* out, int n, const double* const* xs, c
   std::memset(&c, 0, sizeof(c));
   c.n = n;
   for (int k = 0; k < n; ++k) { c.t[k].x = xs[k]; c.t[k].m1 = m1[k]; c.t[k].den = den[k]; c.t[k].neg = neg[k]; }
-  hipLaunchKernelGGL(k_blincomb, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, b->stream, b->total, out, c);
+  hipLaunchKernelGGL(k_blincomb, dim3(stride_blocks(b->total, 256)), dim3(256), 0, b->stream, b->total, out, c);
   BHIP(b, hipGetLastError());
   return DGV_OK;
 }
@@ -791,7 +793,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
       const int64_t n = pb.npairs * (D + 1);
       if (hipMalloc(&b->d_pairs32[g], sizeof(float) * std::max<int64_t>(n, 1)))
         return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 coefficients)");
-      hipLaunchKernelGGL(k_pairs_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, pb.npairs, D,
+      hipLaunchKernelGGL(k_pairs_to_f32, dim3(stride_blocks(n, 256)), dim3(256), 0, b->stream, pb.npairs, D,
                          pb.d_pairs, b->d_pairs32[g]);
     }
     gd[g].pairs64 = pb.d_pairs;

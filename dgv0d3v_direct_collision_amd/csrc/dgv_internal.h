@@ -33,6 +33,15 @@ namespace dgv {
 
 constexpr int kMaxS = 8;
 
+// Blocks for a grid-stride loop over n items. HIP limits one grid dimension to
+// fewer than 2^32 work-items, and the coefficient arrays of a 33^3 grid already
+// hold more elements than that, so element-wise kernels stride over a capped grid.
+inline unsigned stride_blocks(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  if (b > (1 << 20)) b = 1 << 20;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
 struct Grid {
   int Mu = 0, Mv = 0, Mw = 0, su = 0, sv = 0, sw = 0, D = 0;
   double box[6] = {0, 0, 0, 0, 0, 0};

@@ -556,19 +556,20 @@ int launch_copy16(dgv_ctx* ctx, void* dst, const void* src, size_t bytes) {
 }
 
 __global__ void k_lincomb(int64_t n, double* __restrict__ out, LinComb c) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < c.n; ++k) {
-    const LinTerm& t = c.t[k];
-    double v = t.x[i];
-    if (t.m1 != 1.0) v = v * t.m1;
-    if (t.m2 != 1.0) v = v * t.m2;
-    if (t.den != 1.0) v = v / t.den;
-    if (k == 0) s = t.neg ? -v : v;
-    else s = t.neg ? s - v : s + v;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    double s = 0.0;
+    for (int k = 0; k < c.n; ++k) {
+      const LinTerm& t = c.t[k];
+      double v = t.x[i];
+      if (t.m1 != 1.0) v = v * t.m1;
+      if (t.m2 != 1.0) v = v * t.m2;
+      if (t.den != 1.0) v = v / t.den;
+      if (k == 0) s = t.neg ? -v : v;
+      else s = t.neg ? s - v : s + v;
+    }
+    out[i] = s;
   }
-  out[i] = s;
 }
 
 int launch_lincomb(dgv_ctx* ctx, int64_t n, double* out, int nterms, const double* const* xs, const double* m1,
@@ -583,7 +584,7 @@ int launch_lincomb(dgv_ctx* ctx, int64_t n, double* out, int nterms, const doubl
     c.t[k].den = den ? den[k] : 1.0;
     c.t[k].neg = neg ? neg[k] : 0;
   }
-  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n, out, c);
+  hipLaunchKernelGGL(k_lincomb, dim3(stride_blocks(n, 256)), dim3(256), 0, ctx->stream, n, out, c);
   DGV_HIP(ctx, hipGetLastError());
   return DGV_OK;
 }

@@ -57,11 +57,12 @@ __global__ __launch_bounds__(256) void k_truncate(const BlockHdr* __restrict__ b
 
 __global__ void k_gather_pairs(int64_t npairs, int D, const double* __restrict__ src,
                                const int32_t* __restrict__ newpos, double* __restrict__ dst) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npairs * (D + 1)) return;
-  const int64_t p = i / (D + 1), l = i % (D + 1);
-  const int32_t np = newpos[p];
-  if (np >= 0) dst[(int64_t)np * (D + 1) + l] = src[i];
+  const int64_t n = npairs * (D + 1), step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const int64_t p = i / (D + 1), l = i % (D + 1);
+    const int32_t np = newpos[p];
+    if (np >= 0) dst[(int64_t)np * (D + 1) + l] = src[i];
+  }
 }
 
 __global__ void k_count_nonzero(int64_t npairs, int D, const double* __restrict__ pairs,
@@ -127,7 +128,7 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   e = hipMemcpyAsync(d_pos, newpos.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream);
   const int64_t tot = np * (D + 1);
   if (e == hipSuccess && tot > 0)
-    hipLaunchKernelGGL(k_gather_pairs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, np, D,
+    hipLaunchKernelGGL(k_gather_pairs, dim3(stride_blocks(tot, 256)), dim3(256), 0, ctx->stream, np, D,
                        pb.d_pairs, d_pos, d_new);
   if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -171,9 +172,9 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
 }
 
 __global__ void k_abs_coeff(int64_t npairs, int D, double* __restrict__ pairs) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npairs * (D + 1)) return;
-  if (i % (D + 1) < D) pairs[i] = fabs(pairs[i]);
+  const int64_t n = npairs * (D + 1), step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+    if (i % (D + 1) < D) pairs[i] = fabs(pairs[i]);
 }
 
 }  // namespace dgv
@@ -188,7 +189,7 @@ int dgv_ctx_aarray_abs(dgv_ctx* ctx) {
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t tot = ctx->pb.npairs * (ctx->g.D + 1);
   if (tot > 0)
-    hipLaunchKernelGGL(k_abs_coeff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, ctx->pb.npairs,
+    hipLaunchKernelGGL(k_abs_coeff, dim3(stride_blocks(tot, 256)), dim3(256), 0, ctx->stream, ctx->pb.npairs,
                        ctx->g.D, ctx->pb.d_pairs);
   DGV_HIP(ctx, hipGetLastError());
   DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -1,0 +1,62 @@
+"""A 33^3-node grid (M=11, 3 nodes per dimension, box [-4.5, 4.5]^3), whose
+pair-coefficient array holds more than 2^32 elements (155M pairs x 28 values).
+
+Element-wise kernels over that array must reach every element: HIP limits one
+grid dimension to fewer than 2^32 work-items, so a one-thread-per-element launch
+would silently cover only the first part. Checked at full size through
+size-independent properties:
+ * FP64 batch == single-VDF evaluator (relative L2 <= 1e-12);
+ * FP32-compensated batch against FP64, per node, within 4 * 2^-24 of the
+   term scale 2/w sum |A| f f (its FP32 coefficients are converted element-wise);
+ * the |A| context (element-wise fabs) obeys the triangle inequality
+   |Q(f)| <= Q_|A|(f) at every node for f >= 0.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_33cubed_grid_past_2_32_coefficients():
+    import dgv0d3v_direct_collision_amd as dgv
+    print("\n[large] grid M=11 s=3 on [-4.5,4.5]^3", flush=True)
+    g = dgv.DGV(11, 3, (-4.5, 4.5))
+    g.generate_aarray(min_sens=0.0)
+    wl = g.workload()
+    print(f"[large] pairs {wl['pairs']}, coefficient elements {wl['pairs'] * 28}", flush=True)
+    assert wl["pairs"] * 28 > 2 ** 32
+    rng = np.random.default_rng(33)
+    u, v, w = g.nodes_u, g.nodes_v, g.nodes_w
+    f = np.zeros(g.nnodes)
+    for k in range(3):
+        n, T = rng.uniform(0.3, 1.0), rng.uniform(0.4, 1.0)
+        ub = rng.uniform(-0.8, 0.8, 3)
+        f += n * np.exp(-((u - ub[0]) ** 2 + (v - ub[1]) ** 2 + (w - ub[2]) ** 2) / T) / (np.pi * T) ** 1.5
+    q_single = g.EvalCollisionPeriodicAPlus_DGV(f)
+    qs = {}
+    for prec in ("FP64", "FP32_COMP"):
+        b = dgv.DGVBatch([g], np.zeros(4, np.int32), getattr(dgv, "DGV_" + prec))
+        for i in range(4):
+            b.set_f(i, f)
+        b.eval_q()
+        qs[prec] = [b.get_q(i) for i in range(4)]
+        b.close()
+    for i in range(4):
+        assert rel_l2(qs["FP64"][i], q_single) <= 1e-12
+    g.close()
+    a = dgv.DGV(11, 3, (-4.5, 4.5))
+    a.generate_aarray(min_sens=0.0)
+    a.aarray_abs()
+    qabs = a.EvalCollisionPeriodicAPlus_DGV(f)
+    a.close()
+    slack = np.min(qabs - np.abs(q_single)) / np.max(qabs)
+    print(f"[large] triangle slack min {slack:.3e}", flush=True)
+    assert slack >= -1e-12
+    for i in range(4):
+        err = np.abs(qs["FP32_COMP"][i] - qs["FP64"][i])
+        ratio = np.max(err / np.maximum(qabs, 1e-300)) / 2.0 ** -24
+        print(f"[large] FP32 instance {i}: rel L2 {rel_l2(qs['FP32_COMP'][i], qs['FP64'][i]):.3e}, "
+              f"max error / term scale {ratio:.3f} * 2^-24", flush=True)
+        assert ratio <= 4.0
