@@ -79,6 +79,16 @@ SIGNATURES = {
     "dgv_ctx_set_kernel": (C.c_int, [_vp, C.c_int]),
     "dgv_ctx_set_tile": (C.c_int, [_vp, _i32]),
     "dgv_ctx_set_projection": (C.c_int, [_vp, _i32]),
+    "dgv_prepare_fma": (C.c_int, [_vp, _dp, _i64]),
+    "dgv_prepare_fma_dev": (C.c_int, [_vp, _vp, _i64]),
+    "dgv_eval_linear": (C.c_int, [_vp, _dp, _dp, _i64]),
+    "dgv_eval_linear_dev": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i32]),
+    "dgv_ctx_get_fma": (C.c_int, [_vp, _dp, _i64]),
+    "dgv_ctx_get_fm": (C.c_int, [_vp, _dp, _i64]),
+    "dgv_ctx_set_regime": (C.c_int, [_vp, _i32, _dbl, _dbl]),
+    "dgv_ctx_get_regime": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64)]),
+    "dgv_check_solution_mode": (C.c_int, [_vp, _dp, _dp, _i64]),
     "dgv_ctx_autotune": (C.c_int, [_vp, _ip, _dp]),
     "dgv_spatial_oper": (C.c_int, [_vp, _dp, _dp, _i64, _dbl, _dbl]),
     "dgv_spatial_oper_dev": (C.c_int, [_vp, _vp, _vp, _i64, _dbl, _dbl]),

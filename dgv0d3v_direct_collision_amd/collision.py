@@ -270,8 +270,72 @@ class DGV:
         self._chk(self._L.dgv_eval_mixed(self._ctx, dptr(f), dptr(fm), dptr(fc), self.nnodes))
         return fc
 
+    def PrepareFMA_DGV(self, fm=None):
+        """collision_mod.f90:377-464: the dense linearised operator at the
+        Maxwellian fm (the context's last one when fm is None), kept on the GPU."""
+        if fm is None:
+            fm = self.fm
+        fm = self._f(fm)
+        self._chk(self._L.dgv_prepare_fma(self._ctx, dptr(fm), self.nnodes))
+
+    def EvalCollisionLinear(self, f) -> np.ndarray:
+        """collision_mod.f90:346-359: fc = 2 MATMUL(f, fmA)."""
+        f = self._f(f)
+        fc = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_eval_linear(self._ctx, dptr(f), dptr(fc), self.nnodes))
+        return fc
+
+    def eval_linear_dev(self, d_f: int, d_fc: int, nvec: int = 1, ldf: int = 0, ldq: int = 0):
+        n = self.nnodes
+        self._chk(self._L.dgv_eval_linear_dev(self._ctx, C.c_void_p(int(d_f)), ldf or n, C.c_void_p(int(d_fc)),
+                                              ldq or n, n, nvec))
+
+    @property
+    def fmA(self) -> np.ndarray:
+        """fmA indexed [row, column] as the reference's fmA(row, column)."""
+        n = self.nnodes
+        buf = np.zeros(n * n)
+        self._chk(self._L.dgv_ctx_get_fma(self._ctx, dptr(buf), n))
+        return buf.reshape(n, n).T
+
+    @property
+    def fm(self) -> np.ndarray:
+        """The Maxwellian of the linearised operator (commvar fm)."""
+        out = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_ctx_get_fm(self._ctx, dptr(out), self.nnodes))
+        return out
+
+    def set_regime(self, run_mode: int = 0, decomp_lev: float = 0.0, linear_lev: float = 0.0):
+        """commvar run_mode, decomp_lev, linear_lev.  decomp_lev > 0 turns on
+        CheckSolutionMode_DGV before every spatial operator."""
+        self._chk(self._L.dgv_ctx_set_regime(self._ctx, run_mode, decomp_lev, linear_lev))
+
+    def regime(self) -> dict:
+        """run_mode, the last L1 deviation from the local Maxwellian, the last
+        |fm - f_M|_1 / n, and the numbers of checks and fmA builds."""
+        m = C.c_int32()
+        e = C.c_double()
+        efm = C.c_double()
+        cnt = (C.c_int64 * 2)()
+        self._chk(self._L.dgv_ctx_get_regime(self._ctx, C.byref(m), C.byref(e), C.byref(efm), cnt))
+        return {"run_mode": m.value, "l1_err": e.value, "l1_err_fm": efm.value, "checks": cnt[0],
+                "fma_builds": cnt[1]}
+
+    @property
+    def run_mode(self) -> int:
+        return self.regime()["run_mode"]
+
+    def CheckSolutionMode_DGV(self, f) -> np.ndarray:
+        """spat_oper_mod.f90:195-272: regime switch; returns Df = f - fm
+        (zeros when the solution is in run_mode 0)."""
+        f = self._f(f)
+        df = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_check_solution_mode(self._ctx, dptr(f), dptr(df), self.nnodes))
+        return df
+
     def SpatialOperSH_DGV(self, f, dt: float) -> np.ndarray:
-        """spat_oper_mod.f90:28-68, run_mode 0"""
+        """spat_oper_mod.f90:28-68: run_mode 0, 1 or 2 after the regime check
+        when it is on (set_regime)."""
         f = self._f(f)
         fcol = np.zeros(self.nnodes)
         self._chk(self._L.dgv_spatial_oper(self._ctx, dptr(f), dptr(fcol), self.nnodes, dt,

@@ -113,6 +113,7 @@ void dgv_ctx_destroy(dgv_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_plan(ctx);
   free_pairblock(ctx);
+  free_linear_vectors(ctx);
   hipFree(ctx->d_gwts); hipFree(ctx->d_u); hipFree(ctx->d_v); hipFree(ctx->d_w);
   hipFree(ctx->d_f); hipFree(ctx->d_q); hipFree(ctx->d_red);
   for (int i = 0; i < 6; ++i) hipFree(ctx->d_tmp[i]);
@@ -397,7 +398,7 @@ int dgv_ctx_autotune(dgv_ctx* ctx, int32_t* best_pairs_per_item, double* best_ms
 int dgv_spatial_oper_dev(dgv_ctx* ctx, const double* d_f, double* d_fcol, int64_t n, double dt, double coef) {
   CHECK_CTX(ctx);
   CHECK_N(ctx, n);
-  return eval_q_dev(ctx, d_f, d_fcol, coef * dt);
+  return spatial_oper_device(ctx, d_f, d_fcol, coef * dt);
 }
 
 int dgv_spatial_oper(dgv_ctx* ctx, const double* f, double* fcol, int64_t n, double dt, double coef) {
@@ -407,7 +408,7 @@ int dgv_spatial_oper(dgv_ctx* ctx, const double* f, double* fcol, int64_t n, dou
   const size_t nb = sizeof(double) * n;
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
-  int rc = eval_q_dev(ctx, ctx->d_f, ctx->d_q, coef * dt);
+  int rc = spatial_oper_device(ctx, ctx->d_f, ctx->d_q, coef * dt);
   if (rc) return rc;
   DGV_HIP(ctx, hipMemcpyAsync(fcol, ctx->d_q, nb, hipMemcpyDeviceToHost, ctx->stream));
   DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -427,7 +428,7 @@ static int rk_step_device(dgv_ctx* ctx, double* f, double dt, double coef, int r
   double* tmp = ctx->d_tmp[4];
   const double ct = coef * dt;
   int rc;
-#define SOP(in, out) do { rc = eval_q_dev(ctx, in, out, ct); if (rc) return rc; } while (0)
+#define SOP(in, out) do { rc = spatial_oper_device(ctx, in, out, ct); if (rc) return rc; } while (0)
 #define LC(out, N, ...)                                                                  \
   do {                                                                                   \
     const double* xs_[] = {__VA_ARGS__};                                                 \
@@ -602,7 +603,9 @@ int dgv_rk_run_dev(dgv_ctx* ctx, double* d_f, int64_t n, double dt, double coef,
     if (rc) return rc;
   }
   DGV_HIP(ctx, hipSetDevice(ctx->device));
-  if (!use_graph || rkmts == 5) {  // RK5 swaps scratch buffers between steps
+  // RK5 swaps scratch buffers between steps; the regime check decides on the
+  // host at every stage
+  if (!use_graph || rkmts == 5 || ctx->lin.decomp_lev > 0.0) {
     for (int64_t s = 0; s < nsteps; ++s) {
       int rc = rk_step_device(ctx, d_f, dt, coef, rkmts, nullptr);
       if (rc) return rc;
@@ -653,7 +656,7 @@ int dgv_ab_step(dgv_ctx* ctx, double* f, int64_t n, double dt, double coef, int3
   DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
   for (int k = 1; k < rkmts; ++k)
     DGV_HIP(ctx, hipMemcpyAsync(d_r[k], h[4 + k], nb, hipMemcpyHostToDevice, ctx->stream));
-  int rc = eval_q_dev(ctx, ctx->d_f, d_r[0], coef * dt);
+  int rc = spatial_oper_device(ctx, ctx->d_f, d_r[0], coef * dt);
   if (rc) return rc;
   const double* xs[6] = {ctx->d_f, d_r[0], d_r[1], d_r[2], d_r[3], d_r[4]};
   double m1[6] = {1, 1, dt, dt, dt, dt}, m2[6] = {1, 1, 1, 1, 1, 1}, den[6] = {1, 1, 1, 1, 1, 1};

@@ -133,6 +133,7 @@ int build_grid(const dgv_grid_desc& d, Grid& g, std::string& err) {
 }
 
 void free_pairblock(dgv_ctx* ctx) {
+  free_linear(ctx);
   PairBlock& pb = ctx->pb;
   hipFree(pb.d_pairs); hipFree(pb.d_blocks); hipFree(pb.d_group_blocks); hipFree(pb.d_groups);
   pb.d_pairs = nullptr; pb.d_blocks = nullptr; pb.d_group_blocks = nullptr; pb.d_groups = nullptr;

@@ -102,6 +102,26 @@ struct Plan {
   int64_t ncell_rows = 0;
 };
 
+struct FmaItem;
+
+// Decomposition / linearised regimes (PrepareFMA_DGV, EvalCollisionLinear,
+// CheckSolutionMode_DGV).  Built on first use, dropped with the A-array.
+struct LinearState {
+  double* d_fma = nullptr;        // N x N, fmA(r, i) at r * N + i
+  int16_t* d_pidx = nullptr;      // per block: pair index of (p, q) or -1
+  FmaItem* d_items = nullptr;
+  int32_t* d_l1 = nullptr;        // block lists by a-offset / b-offset
+  int32_t* d_l2 = nullptr;
+  double* d_part = nullptr;       // GEMV partial sums
+  int64_t nitems = 0;
+  bool fma_valid = false;
+  double* d_vec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // fm, new Maxwellian, Df, scratch, reduction
+  int32_t run_mode = 0;
+  double decomp_lev = 0.0, linear_lev = 0.0;
+  double last_l1 = 0.0, last_l1_fm = 0.0;
+  int64_t nchecks = 0, nprepare = 0;
+};
+
 }  // namespace dgv
 
 struct dgv_ctx {
@@ -136,6 +156,7 @@ struct dgv_ctx {
   double rk_graph_dt = 0, rk_graph_coef = 0;
   double* rk_graph_f = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  dgv::LinearState lin;
   // host copy of the reference layout, built on first export
   bool exp_valid = false;
   std::vector<std::vector<std::array<int64_t, 2>>> exp_idx;
@@ -185,6 +206,14 @@ int device_project(dgv_ctx* ctx, double* d_q);
 // Cholesky factor (row-major, lower) of the Gram matrix of psi = 1,u,v,w,|u|^2
 // in the w-weighted inner product (dgv_capi.cpp)
 int gram_cholesky(const Grid& g, double L[25]);
+
+// decomposition / linearised regimes (dgv_linear.hip)
+void free_linear(dgv_ctx* ctx);
+void free_linear_vectors(dgv_ctx* ctx);
+int prepare_fma_dev(dgv_ctx* ctx, const double* d_fm);
+int eval_linear_dev(dgv_ctx* ctx, const double* d_f, int64_t ldf, double* d_fc, int64_t ldq, int nvec);
+int check_solution_mode(dgv_ctx* ctx, const double* d_f);
+int spatial_oper_device(dgv_ctx* ctx, const double* d_f, double* d_out, double scale);
 
 // generator (dgv_agen.hip)
 int generate_pairblock(dgv_ctx* ctx, const dgv_agen_params& p);

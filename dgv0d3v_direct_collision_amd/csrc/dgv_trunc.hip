@@ -81,6 +81,7 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array");
   PairBlock& pb = ctx->pb;
   const int D = ctx->g.D;
+  free_linear(ctx);  // fmA and its tables belong to the old A-array
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const int64_t np = pb.npairs;

@@ -151,8 +151,42 @@ int dgv_ctx_set_tile(dgv_ctx* ctx, int32_t pairs_per_item);
 int dgv_ctx_set_projection(dgv_ctx* ctx, int32_t on);
 int dgv_ctx_autotune(dgv_ctx* ctx, int32_t* best_pairs_per_item, double* best_ms);
 
-/* SpatialOperSH_DGV(f,fcol,dt), run_mode 0 (spat_oper_mod.f90:28-68):
- * fcol = Q(f) * coef * dt with coef = (mol_diam/L_inf)^2 * N_inf. */
+/* ---- decomposition and linearised regimes ---------------------------------------- */
+/* PrepareFMA_DGV (collision_mod.f90:377-464): builds the dense linearised
+ * operator fmA(r, i) = (1/w_i) sum over node i's shifted records of
+ * fm(xi1) A at r = xi and fm(xi) A at r = xi1, N x N doubles in device memory
+ * (3.1 GB at 27^3).  fm becomes the context's Maxwellian of the regime check. */
+int dgv_prepare_fma(dgv_ctx* ctx, const double* fm, int64_t n);
+int dgv_prepare_fma_dev(dgv_ctx* ctx, const double* d_fm, int64_t n);
+/* EvalCollisionLinear(f,fc) (collision_mod.f90:346-359): fc = 2 MATMUL(f, fmA).
+ * The _dev form applies it to nvec vectors (f at d_f + k*ldf, fc at d_fc + k*ldq). */
+int dgv_eval_linear(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
+int dgv_eval_linear_dev(dgv_ctx* ctx, const double* d_f, int64_t ldf, double* d_fc, int64_t ldq,
+                        int64_t n, int32_t nvec);
+/* fmA in the reference's column-major layout: fmA(r, i) at fmA[r + n*i] (n*n doubles) */
+int dgv_ctx_get_fma(dgv_ctx* ctx, double* fmA, int64_t n);
+/* the Maxwellian fmA was last built at (commvar fm) */
+int dgv_ctx_get_fm(dgv_ctx* ctx, double* fm, int64_t n);
+/* run_mode, decomp_lev, linear_lev of commvar (parameters.dat).  decomp_lev > 0
+ * turns on CheckSolutionMode_DGV before every spatial operator, as the
+ * reference's SpatialOperSH_DGV calls it; decomp_lev <= 0 keeps run_mode 0 (the
+ * reference's check then always lands in run_mode 0).  run_mode 1 or 2 needs a
+ * prepared fmA.  With switching on, RK runs step eagerly (no graph replay). */
+int dgv_ctx_set_regime(dgv_ctx* ctx, int32_t run_mode, double decomp_lev, double linear_lev);
+/* current run_mode, the last relative L1 deviation from the local Maxwellian
+ * (the value KeepTrackL1_err_DGV records), the last |fm - f_M|_1 / n, and
+ * counts = {regime checks, fmA builds} */
+int dgv_ctx_get_regime(dgv_ctx* ctx, int32_t* run_mode, double* l1_err, double* l1_err_fm,
+                       int64_t counts[2]);
+/* CheckSolutionMode_DGV(f,Df) (spat_oper_mod.f90:195-272): moments, local
+ * Maxwellian, L1 deviation, regime switch (rebuilding fmA as the reference
+ * does), Df = f - fm (zeros in run_mode 0; df may be NULL). */
+int dgv_check_solution_mode(dgv_ctx* ctx, const double* f, double* df, int64_t n);
+
+/* SpatialOperSH_DGV(f,fcol,dt) (spat_oper_mod.f90:28-68): fcol = C(f) * coef * dt
+ * with coef = (mol_diam/L_inf)^2 * N_inf and C = Q(f) (run_mode 0),
+ * L(Df) + Q(Df) (run_mode 1) or L(Df) (run_mode 2), after the regime check when
+ * it is enabled (dgv_ctx_set_regime). */
 int dgv_spatial_oper(dgv_ctx* ctx, const double* f, double* fcol, int64_t n, double dt,
                      double coef);
 int dgv_spatial_oper_dev(dgv_ctx* ctx, const double* d_f, double* d_fcol, int64_t n,

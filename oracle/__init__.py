@@ -24,6 +24,8 @@ function               reference routine restated
 ``eval_mixed``          EvalCollisionPeriodicMixedTermsA_DGV (collision_mod.f90:244-332)
 ``prepare_fma``         PrepareFMA_DGV (collision_mod.f90:377-464)
 ``eval_linear``         EvalCollisionLinear (collision_mod.f90:346-359)
+``Regime``              CheckSolutionMode_DGV and the run_mode switch of
+                       SpatialOperSH_DGV (spat_oper_mod.f90:195-272, 28-68)
 ``trunc_threshold``     TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362)
 ``trunc_radius``        TruncAarrsRadius_DGV (dgvtools_mod.f90:1375-1429)
 ``seta`` / ``seta_raw``  SetA_DGV with the Lagrange basis restored, its adaptive
@@ -373,3 +375,65 @@ def eval_linear(fmA, f):
     fc = np.zeros(N)
     lib().orc_eval_linear(N, colmajor, _c(f, np.float64), fc)
     return fc
+
+
+class Regime:
+    """CheckSolutionMode_DGV (spat_oper_mod.f90:195-272) and the run_mode
+    switch of SpatialOperSH_DGV (spat_oper_mod.f90:41-60) over the oracle's
+    evaluators.  ``collision(f)`` is the unscaled collision term of one
+    SpatialOperSH_DGV call; pass it as ``q`` to ``rk_step`` / ``ab_step``.
+    ``trace`` holds (run_mode after the check, L1_err) per call."""
+
+    def __init__(self, G: "Geometry", capphi, A, A_xi, A_xi1, decomp_lev, linear_lev, run_mode=0):
+        self.G = G
+        self.A = (capphi, A, A_xi, A_xi1)
+        self.decomp_lev = decomp_lev
+        self.linear_lev = linear_lev
+        self.run_mode = run_mode
+        self.fm = None
+        self.fmA = None
+        self.fma_builds = 0
+        self.trace = []
+
+    def _prepare(self, fmn):
+        self.fm = fmn.copy()                                   # fm = fMaxwellNew
+        self.fmA = prepare_fma(self.G, *self.A, self.fm)       # PrepareFMA_DGV
+        self.fma_builds += 1
+
+    def check(self, f):
+        """Returns Df (None in run_mode 0)."""
+        G = self.G
+        gw = G.nodes_gwts
+        dens, ub, vb, wb, T = masscheck(G, f)                  # line 236
+        fmn = np.array([maxwell(T, ub, vb, wb, dens, u, v, w)
+                        for u, v, w in zip(G.nodes_u, G.nodes_v, G.nodes_w)])
+        l1 = float(np.sum(np.abs(f - fmn) * gw) / dens)        # line 238
+        df = None
+        if l1 < self.decomp_lev:
+            if self.run_mode == 0:
+                self._prepare(fmn)
+                self.run_mode = 2 if l1 < self.linear_lev else 1
+            elif self.run_mode in (1, 2):
+                l1_fm = float(np.sum(np.abs(self.fm - fmn) * gw) / dens)
+                if l1_fm > self.linear_lev / 2.0:
+                    self._prepare(fmn)
+                if self.run_mode == 1 and l1 < self.linear_lev:
+                    self.run_mode = 2
+                elif self.run_mode == 2 and l1 > self.linear_lev:
+                    self.run_mode = 1
+            else:
+                raise ValueError("run_mode")
+            df = f - self.fm                                   # line 264
+        else:
+            self.run_mode = 0
+        self.trace.append((self.run_mode, l1))
+        return df
+
+    def collision(self, f):
+        df = self.check(f)
+        if self.run_mode == 0:
+            return eval_aplus(self.G, *self.A, f, nthreads=8)
+        lin = eval_linear(self.fmA, df)
+        if self.run_mode == 1:
+            return eval_aplus(self.G, *self.A, df, nthreads=8) + lin
+        return lin
