@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-linear", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the mixed-grid batching and 33^3 precision configurations")
     ap.add_argument("--sur-rows", type=int, default=2048, help="(f, Q) pairs generated for the surrogate")
@@ -244,6 +245,11 @@ def main():
               "projected": rk3_run(d, d_f, f, args.rk_steps, project=True)}
         d.set_projection(False)
 
+    # ---- linearised regime ------------------------------------------------------------------
+    lin = None
+    if not args.no_linear:
+        lin = linear_leg(d, f, u, v, w, gw, rng)
+
     # ---- the batching and precision configurations ------------------------------------------
     mixed = prec33 = None
     if not args.no_configs:
@@ -319,6 +325,7 @@ def main():
                          "after_projection_rel_max": proj_rel},
         "rk3": rk,
         "surrogate": sur,
+        "linearised_regime": lin,
         "mixed_grid_batch": mixed,
         "precision_33cubed": prec33,
     }
@@ -427,6 +434,72 @@ def precision_leg(dgv, rng, args):
     return {"nodes": 35937, "vdfs": 64, "per_perturbation_amplitude": per,
             "fp32comp_vdf_q_per_s": 64 / times["FP32_COMP"], "fp64_vdf_q_per_s": 64 / times["FP64"],
             "projected_moment_rel_max": proj}
+
+
+def linear_leg(d, f, u, v, w, gw, rng):
+    """The linearised regime at 27^3 (PrepareFMA_DGV, EvalCollisionLinear,
+    CheckSolutionMode_DGV): the dense fmA built at the local Maxwellian of f,
+    the GEMV against its HBM roofline (one fmA read per pass of up to 8
+    vectors), the linear term against the sparse mixed term, and RK3 steps that
+    run the regime check before every stage, in run_mode 2 and in run_mode 1."""
+    N = d.nnodes
+    dens, ub, vb, wb, T = d.MassCheckRec(f)
+    fm = maxwellian(u, v, w, dens, (ub, vb, wb), T)
+    t = time.perf_counter()
+    d.PrepareFMA_DGV(fm)
+    t_first = time.perf_counter() - t
+    t = time.perf_counter()
+    d.PrepareFMA_DGV(fm)
+    t_build = time.perf_counter() - t
+    df = f - fm
+    lin = d.EvalCollisionLinear(df)
+    err = rel_l2(lin, d.EvalCollisionPeriodicMixedTermsA_DGV(df, fm))
+    log(f"linear: fmA build {t_build * 1e3:.1f} ms (first, with tables: {t_first * 1e3:.1f} ms); "
+        f"L(Df) vs Mixed(Df, fm) rel L2 {err:.2e}")
+    X = np.tile(df, (8, 1))
+    d_x = d.device_alloc(X.nbytes)
+    d_y = d.device_alloc(X.nbytes)
+    d.htod(d_x, X)
+    gemv = {}
+    for nv, reps in ((1, 40), (8, 10)):
+        d.eval_linear_dev(d_x, d_y, nvec=nv)
+        d.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            d.eval_linear_dev(d_x, d_y, nvec=nv)
+        d.synchronize()
+        ms = (time.perf_counter() - t) / reps * 1e3
+        gbs = 8.0 * N * N / (ms * 1e-3) / 1e9
+        gemv[f"nvec{nv}"] = {"ms_per_pass": ms, "vdf_per_s": nv / (ms * 1e-3), "fma_read_gbs": gbs,
+                             "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+        log(f"linear: EvalCollisionLinear, {nv} vector(s) per pass: {ms:.3f} ms, fmA read at {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.2f} of peak)")
+    d.device_free(d_x)
+    d.device_free(d_y)
+    # RK3 with the regime check before every stage, from a near-equilibrium f
+    fnear = fm * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, N))
+    fnear = fnear / np.sum(gw * fnear) * dens
+    runs = {}
+    # run_mode 1 with linear_lev 0.01: the L1 deviation (about 0.015-0.02) keeps it
+    # there, and fmA is rebuilt only when the Maxwellian moves by more than 0.005
+    for name, dlev, llev, nsteps in (("run_mode_2", 0.5, 0.2, 60), ("run_mode_1", 0.5, 0.01, 10)):
+        d.set_regime(0, dlev, llev)
+        fr = fnear.copy()
+        d.rkmts = 3
+        d.TimeIntegratorORK_SH_DGV(fr, 0.005)
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            d.TimeIntegratorORK_SH_DGV(fr, 0.005)
+        sec = time.perf_counter() - t
+        st = d.regime()
+        runs[name] = {"steps": nsteps, "steps_per_sec": nsteps / sec, "run_mode": st["run_mode"],
+                      "l1_err": st["l1_err"], "fma_builds_total": st["fma_builds"],
+                      "density_drift": float(d.MassCheckRec(fr)[0] - dens)}
+        log(f"linear: RK3 with the regime check, {name}: {nsteps / sec:.1f} steps/s, run_mode {st['run_mode']}, "
+            f"L1 {st['l1_err']:.3e}")
+    d.set_regime(0, 0.0, 0.0)
+    return {"nodes": N, "fma_bytes": 8 * N * N, "prepare_fma_ms": t_build * 1e3,
+            "prepare_fma_first_ms": t_first * 1e3, "linear_vs_mixed_rel_l2": err, "gemv": gemv, "rk3": runs}
 
 
 def rk3_run(d, d_f, f, nsteps, project):

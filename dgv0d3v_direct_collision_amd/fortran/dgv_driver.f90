@@ -1,13 +1,16 @@
 ! dgv_driver.f90 -- serial driver in the shape of DGVblzmPL.f90's time loop,
 ! on the GPU through the dgv_gpu shim: grid, A-array, one Q(f), nsteps RK steps.
-! Usage: dgv_driver M box rkmts nsteps dt out.bin
-! Writes f0, Q(f0), f after the steps (raw float64) to out.bin.
+! Usage: dgv_driver M box rkmts nsteps dt out.bin [decomp_lev linear_lev nsteps2]
+! Writes f0, Q(f0), f after the steps (raw float64) to out.bin.  With the three
+! optional arguments it then turns on the regime switch (CheckSolutionMode_DGV
+! before every spatial operator), runs nsteps2 more steps and appends f,
+! run_mode and EvalCollisionLinear(f) at the last Maxwellian.
 program dgv_driver
   use dgv_gpu
   implicit none
   integer, parameter :: DP = kind(1.0d0)
   character(len=256) :: arg, outp
-  integer :: M, nsteps, i, n, u
+  integer :: M, nsteps, nsteps2, i, n, u
   real(DP) :: box, dt, dens, ub, vb, wb, T
   real(DP), allocatable :: f(:), fc(:), x(:)
   call get_command_argument(1, arg); read (arg, *) M
@@ -33,6 +36,18 @@ program dgv_driver
     call TimeIntegratorORK_SH_DGV(f, dt)
   end do
   write (u) f
+  if (command_argument_count() >= 9) then
+    call get_command_argument(7, arg); read (arg, *) decomp_lev
+    call get_command_argument(8, arg); read (arg, *) linear_lev
+    call get_command_argument(9, arg); read (arg, *) nsteps2
+    do i = 1, nsteps2
+      call TimeIntegratorORK_SH_DGV(f, dt)
+    end do
+    print *, "run_mode", run_mode
+    write (u) f, real(run_mode, DP)
+    call EvalCollisionLinear(f, fc)
+    write (u) fc
+  end if
   close (u)
   call dgv_gpu_finalize()
 end program dgv_driver

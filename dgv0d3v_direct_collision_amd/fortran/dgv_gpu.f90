@@ -10,6 +10,13 @@
 !   TimeIntegratorMTS_SH_DGV(f,dt)        time_integr_mod.f90:314-364
 !   PrepareMTS_SH_DGV(f,dt)               time_integr_mod.f90:443-569
 !   MassCheckRec(f,n,ubar,vbar,wbar,temp) dgvtools_mod.f90:1437-1458
+!   EvalCollisionPeriodicMixedTermsA_DGV(f,fm,fc)  collision_mod.f90:244-332
+!   PrepareFMA_DGV                        collision_mod.f90:377-464 (module fm)
+!   EvalCollisionLinear(f,fc)             collision_mod.f90:346-359
+!   CheckSolutionMode_DGV(f,Df)           spat_oper_mod.f90:195-272
+! run_mode, decomp_lev, linear_lev and fm are module variables here (commvar in
+! the reference); SpatialOperSH_DGV and the steppers honour the regime switch
+! when decomp_lev > 0, and leave run_mode and fm as the last check set them.
 ! The grid and A-array that the reference keeps in commvar are handed over once
 ! with dgv_gpu_init and dgv_gpu_set_aarray (or generated with
 ! dgv_gpu_generate_aarray).  A nonzero status from the library prints the
@@ -39,10 +46,17 @@ module dgv_gpu
   real(DP), save, public :: e_time = 0.0_DP
   real(DP), allocatable, save, public :: frhs1(:)
   real(DP), allocatable, save :: hist(:,:)      ! f1..f4, frhs1..frhs5
+  ! decomposition / linearised regimes
+  integer, save, public :: run_mode = 0
+  real(DP), save, public :: decomp_lev = 0.0_DP, linear_lev = 0.0_DP
+  real(DP), allocatable, save, public :: fm(:)
+  integer, save :: pushed_mode = -1
+  real(DP), save :: pushed_decomp = -1.0_DP, pushed_linear = -1.0_DP
 
   public :: dgv_gpu_init, dgv_gpu_set_aarray, dgv_gpu_generate_aarray, dgv_gpu_finalize
   public :: EvalCollisionPeriodicAPlus_DGV, EvalCollisionPeriodicA_DGV, SpatialOperSH_DGV
   public :: TimeIntegratorORK_SH_DGV, TimeIntegratorMTS_SH_DGV, PrepareMTS_SH_DGV, MassCheckRec
+  public :: EvalCollisionPeriodicMixedTermsA_DGV, PrepareFMA_DGV, EvalCollisionLinear, CheckSolutionMode_DGV
 
   interface
     integer(c_int) function c_ctx_create(grid, device, out) bind(C, name="dgv_ctx_create")
@@ -123,6 +137,52 @@ module dgv_gpu
       real(c_double), intent(in) :: f(*)
       integer(c_int64_t), value :: n
       real(c_double), intent(out) :: out(5)
+    end function
+    integer(c_int) function c_eval_mixed(c, f, g, fc, n) bind(C, name="dgv_eval_mixed")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*), g(*)
+      real(c_double), intent(out) :: fc(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_prepare_fma(c, g, n) bind(C, name="dgv_prepare_fma")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: g(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_eval_linear(c, f, fc, n) bind(C, name="dgv_eval_linear")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: fc(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_set_regime(c, mode, dlev, llev) bind(C, name="dgv_ctx_set_regime")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: mode
+      real(c_double), value :: dlev, llev
+    end function
+    integer(c_int) function c_get_regime(c, mode, l1, l1fm, counts) bind(C, name="dgv_ctx_get_regime")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), intent(out) :: mode
+      real(c_double), intent(out) :: l1, l1fm
+      integer(c_int64_t), intent(out) :: counts(2)
+    end function
+    integer(c_int) function c_check_mode(c, f, df, n) bind(C, name="dgv_check_solution_mode")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: df(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_get_fm(c, g, n) bind(C, name="dgv_ctx_get_fm")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: g(*)
+      integer(c_int64_t), value :: n
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
@@ -206,24 +266,84 @@ contains
     real(DP), dimension(:), intent(in) :: f
     real(DP), dimension(:), intent(out) :: fcol
     real(DP), intent(in) :: dt
+    call regime_in("SpatialOperSH_DGV")
     call check(c_spatial_oper(ctx, f, fcol, size(f, kind=c_int64_t), dt, (mol_diam/L_inf)**2*N_inf), &
                "SpatialOperSH_DGV")
+    call regime_out("SpatialOperSH_DGV")
+  end subroutine
+
+  ! module run_mode / levels -> the library, when the driver changed them
+  subroutine regime_in(where)
+    character(*), intent(in) :: where
+    if (run_mode /= pushed_mode .or. decomp_lev /= pushed_decomp .or. linear_lev /= pushed_linear) then
+      call check(c_set_regime(ctx, int(run_mode, c_int32_t), decomp_lev, linear_lev), where)
+      pushed_mode = run_mode; pushed_decomp = decomp_lev; pushed_linear = linear_lev
+    end if
+  end subroutine
+
+  ! the library's run_mode and Maxwellian -> the module, after a regime check
+  subroutine regime_out(where)
+    character(*), intent(in) :: where
+    integer(c_int32_t) :: mode
+    real(c_double) :: l1, l1fm
+    integer(c_int64_t) :: counts(2)
+    if (.not. (decomp_lev > 0.0_DP)) return
+    call check(c_get_regime(ctx, mode, l1, l1fm, counts), where)
+    if (mode /= 0 .or. counts(2) > 0) then
+      if (.not. allocated(fm)) allocate(fm(nnodes))
+      call check(c_get_fm(ctx, fm, nnodes), where)
+    end if
+    run_mode = mode
+    pushed_mode = run_mode
+  end subroutine
+
+  subroutine EvalCollisionPeriodicMixedTermsA_DGV(f, fmx, fc)
+    real(DP), dimension(:), intent(in) :: f, fmx
+    real(DP), dimension(:), intent(out) :: fc
+    call check(c_eval_mixed(ctx, f, fmx, fc, size(f, kind=c_int64_t)), "EvalCollisionPeriodicMixedTermsA_DGV")
+  end subroutine
+
+  ! the linearised operator at the module's fm
+  subroutine PrepareFMA_DGV()
+    if (.not. allocated(fm)) then
+      print *, "dgv_gpu: PrepareFMA_DGV: fm is not set"
+      stop
+    end if
+    call check(c_prepare_fma(ctx, fm, size(fm, kind=c_int64_t)), "PrepareFMA_DGV")
+  end subroutine
+
+  subroutine EvalCollisionLinear(f, fc)
+    real(DP), dimension(:), intent(in) :: f
+    real(DP), dimension(:), intent(out) :: fc
+    call check(c_eval_linear(ctx, f, fc, size(f, kind=c_int64_t)), "EvalCollisionLinear")
+  end subroutine
+
+  subroutine CheckSolutionMode_DGV(f, Df)
+    real(DP), dimension(:), intent(in) :: f
+    real(DP), dimension(:), intent(out) :: Df
+    call regime_in("CheckSolutionMode_DGV")
+    call check(c_check_mode(ctx, f, Df, size(f, kind=c_int64_t)), "CheckSolutionMode_DGV")
+    call regime_out("CheckSolutionMode_DGV")
   end subroutine
 
   ! f is intent(out) in the reference but read there; intent(inout) here
   subroutine TimeIntegratorORK_SH_DGV(f, dt)
     real(DP), dimension(:), intent(inout) :: f
     real(DP), intent(in) :: dt
+    call regime_in("TimeIntegratorORK_SH_DGV")
     call check(c_rk_step(ctx, f, size(f, kind=c_int64_t), dt, (mol_diam/L_inf)**2*N_inf, &
                          int(rkmts, c_int32_t), frhs1), "TimeIntegratorORK_SH_DGV")
+    call regime_out("TimeIntegratorORK_SH_DGV")
   end subroutine
 
   subroutine TimeIntegratorMTS_SH_DGV(f, dt)
     real(DP), dimension(:), intent(inout) :: f
     real(DP), intent(in) :: dt
     hist(:, 5) = frhs1
+    call regime_in("TimeIntegratorMTS_SH_DGV")
     call check(c_ab_step(ctx, f, size(f, kind=c_int64_t), dt, (mol_diam/L_inf)**2*N_inf, &
                          int(rkmts, c_int32_t), hist), "TimeIntegratorMTS_SH_DGV")
+    call regime_out("TimeIntegratorMTS_SH_DGV")
     frhs1 = hist(:, 5)
     e_time = e_time + dt
   end subroutine
