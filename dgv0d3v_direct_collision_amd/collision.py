@@ -81,6 +81,43 @@ def chunk_name(M, s, box, directory="", base="", chunk=0, k_c=2, N_x=3) -> str:
     return buf.value.decode()
 
 
+def sol_file_name(M, s, box, suff, directory="", base="", k_c=1, N_x=1) -> str:
+    """WriteSol_DGV / RestartSolutionDGV file name: MakeBaseNameDGV
+    (readwrite.f90:1505-1560) + "_time" + suff + "_sol.dat"."""
+    g = grid_desc(M, s, box)
+    buf = C.create_string_buffer(1024)
+    check(load().dgv_sol_file_name(C.byref(g), directory.encode(), base.encode(), k_c, N_x, suff.encode(),
+                                   buf, 1024))
+    return buf.value.decode()
+
+
+def log_file_name(M, s, box, suffix, directory="", base="", k_c=1, N_x=1) -> str:
+    """MakeBaseNameDGV + suffix ("_macroerr.txt", "_L1err.txt")."""
+    g = grid_desc(M, s, box)
+    buf = C.create_string_buffer(1024)
+    check(load().dgv_log_file_name(C.byref(g), directory.encode(), base.encode(), k_c, N_x, suffix.encode(),
+                                   buf, 1024))
+    return buf.value.decode()
+
+
+def write_sol_file(path, e_time, dt, rkmts, f, hist=None):
+    """WriteSol_DGV (readwrite.f90:1756-1796); hist = (9, n) f1..f4, frhs1..frhs5."""
+    f = np.ascontiguousarray(f, np.float64)
+    h = None if hist is None else np.ascontiguousarray(hist, np.float64)
+    check(load().dgv_sol_write_file(str(path).encode(), e_time, dt, rkmts, len(f), dptr(f),
+                                    None if h is None else dptr(h)))
+
+
+def read_sol_file(path, n):
+    """RestartSolutionDGV (readwrite.f90:2389-2513) -> (e_time, dt, rkmts, f, hist)."""
+    f = np.zeros(n)
+    hist = np.zeros((9, n))
+    e, dt, k = C.c_double(), C.c_double(), C.c_int32()
+    check(load().dgv_sol_read_file(str(path).encode(), C.byref(e), C.byref(dt), C.byref(k), n, dptr(f),
+                                   dptr(hist)))
+    return e.value, dt.value, k.value, f, hist
+
+
 class DGV:
     """One velocity grid, its A-array and the evaluator state on one GPU."""
 
@@ -389,6 +426,29 @@ class DGV:
         out = np.zeros(5)
         self._chk(self._L.dgv_moments(self._ctx, dptr(f), self.nnodes, dptr(out)))
         return tuple(out)
+
+    def MassCheckRecHighCMoments(self, f):
+        """dgvtools_mod.f90:1501-1549 -> (n, ubar, vbar, wbar, temp, temp_u, temp_v,
+        temp_w, mom3_u, mom3_v, mom3_w, mom4_*, mom5_*, mom6_*)"""
+        f = self._f(f)
+        out = np.zeros(20)
+        self._chk(self._L.dgv_moments_high(self._ctx, dptr(f), self.nnodes, dptr(out)))
+        return tuple(out)
+
+    def WriteSol_DGV(self, path, f, dt: float):
+        """readwrite.f90:1756-1796: e_time, dt, rkmts, f and the multistep
+        history (hist, with frhs1) to a Fortran-unformatted file."""
+        self.hist[4] = self.frhs1
+        write_sol_file(path, self.e_time, dt, self.rkmts, self._f(f), self.hist)
+
+    def RestartSolutionDGV(self, path):
+        """readwrite.f90:2389-2513: restores e_time, rkmts and the history;
+        returns (f, dt)."""
+        e, dt, k, f, hist = read_sol_file(path, self.nnodes)
+        self.e_time, self.rkmts = e, k
+        self.hist = hist
+        self.frhs1 = hist[4].copy()
+        return f, dt
 
     # -- additions --------------------------------------------------------------------
     def collision_moments(self, q):

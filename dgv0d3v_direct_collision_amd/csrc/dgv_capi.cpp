@@ -115,7 +115,7 @@ void dgv_ctx_destroy(dgv_ctx* ctx) {
   free_pairblock(ctx);
   free_linear_vectors(ctx);
   hipFree(ctx->d_gwts); hipFree(ctx->d_u); hipFree(ctx->d_v); hipFree(ctx->d_w);
-  hipFree(ctx->d_f); hipFree(ctx->d_q); hipFree(ctx->d_red);
+  hipFree(ctx->d_f); hipFree(ctx->d_q); hipFree(ctx->d_red); hipFree(ctx->d_hi);
   for (int i = 0; i < 6; ++i) hipFree(ctx->d_tmp[i]);
   for (int i = 0; i < 4; ++i)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
@@ -692,6 +692,23 @@ int dgv_moments(dgv_ctx* ctx, const double* f, int64_t n, double out[5]) {
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
   return device_moments(ctx, ctx->d_f, out);
+}
+
+int dgv_moments_high(dgv_ctx* ctx, const double* f, int64_t n, double out[20]) {
+  CHECK_CTX(ctx);
+  CHECK_N(ctx, n);
+  if (!f || !out) return set_error(ctx, DGV_ERR_ARG, "null array");
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  return device_moments_high(ctx, ctx->d_f, out);
+}
+
+int dgv_moments_high_dev(dgv_ctx* ctx, const double* d_f, int64_t n, double out[20]) {
+  CHECK_CTX(ctx);
+  CHECK_N(ctx, n);
+  if (!d_f || !out) return set_error(ctx, DGV_ERR_ARG, "null array");
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  return device_moments_high(ctx, d_f, out);
 }
 
 int dgv_collision_moments(dgv_ctx* ctx, const double* q, int64_t n, double out[5]) {

@@ -151,6 +151,7 @@ struct dgv_ctx {
   double* d_q = nullptr;
   double* d_tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* d_red = nullptr;        // moment reduction scratch
+  double* d_hi = nullptr;         // high-moment reduction scratch (on first use)
   hipGraphExec_t rk_graph = nullptr;
   int rk_graph_key[2] = {0, 0};
   double rk_graph_dt = 0, rk_graph_coef = 0;
@@ -202,6 +203,7 @@ int launch_axpy(dgv_ctx* ctx, int64_t n, double* out, const double* x0, int nter
 int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);
 int device_moments(dgv_ctx* ctx, const double* d_f, double out[5]);
 int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]);
+int device_moments_high(dgv_ctx* ctx, const double* d_f, double out[20]);
 int device_project(dgv_ctx* ctx, double* d_q);
 // Cholesky factor (row-major, lower) of the Gram matrix of psi = 1,u,v,w,|u|^2
 // in the w-weighted inner product (dgv_capi.cpp)

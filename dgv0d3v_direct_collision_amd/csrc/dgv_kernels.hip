@@ -683,6 +683,85 @@ int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]) {
   return DGV_OK;
 }
 
+// MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549).  After the mass and
+// momentum pass of k_moments_part, one pass of the central sums:
+//   s[0] = sum f w |u - ubar|^2,  s[1 + 3 (k - 2) + d] = sum f w (u_d - ubar_d)^k, k = 2..6.
+constexpr int kHiBlocks = 64;
+constexpr int kHiSums = 16;
+
+__global__ __launch_bounds__(256) void k_high_part(int64_t n, const double* __restrict__ f,
+                                                   const double* __restrict__ w, const double* __restrict__ u,
+                                                   const double* __restrict__ v, const double* __restrict__ ww,
+                                                   const double* __restrict__ mean, double* __restrict__ part) {
+  __shared__ double red[kHiSums][256];
+  double s[kHiSums];
+#pragma unroll
+  for (int k = 0; k < kHiSums; ++k) s[k] = 0.0;
+  const double ub = mean[1], vb = mean[2], wb = mean[3];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kHiBlocks * 256) {
+    const double fw = f[i] * w[i];
+    const double du = u[i] - ub, dv = v[i] - vb, dw = ww[i] - wb;
+    s[0] += fw * (du * du + dv * dv + dw * dw);
+    double pu = du * du, pv = dv * dv, pw = dw * dw;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      s[1 + 3 * k] += fw * pu;
+      s[2 + 3 * k] += fw * pv;
+      s[3 + 3 * k] += fw * pw;
+      pu *= du;
+      pv *= dv;
+      pw *= dw;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kHiSums; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+      for (int k = 0; k < kHiSums; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kHiSums; ++k) part[blockIdx.x * kHiSums + k] = red[k][0];
+}
+
+// out = {n, ubar, vbar, wbar, T, Tu, Tv, Tw, mom3_uvw, mom4_uvw, mom5_uvw, mom6_uvw}
+__global__ void k_high_final(const double* __restrict__ part, const double* __restrict__ mean,
+                             double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[kHiSums];
+  for (int k = 0; k < kHiSums; ++k) s[k] = 0.0;
+  for (int b = 0; b < kHiBlocks; ++b)
+    for (int k = 0; k < kHiSums; ++k) s[k] += part[b * kHiSums + k];
+  const double n = mean[0];
+  for (int k = 0; k < 4; ++k) out[k] = mean[k];
+  out[4] = s[0] / n / 3.0 * 2.0;
+  for (int d = 0; d < 3; ++d) out[5 + d] = s[1 + d] / n / 3.0 * 2.0;  // directional T (lines 1525-1527)
+  for (int k = 1; k < 5; ++k)
+    for (int d = 0; d < 3; ++d) out[8 + 3 * (k - 1) + d] = s[1 + 3 * k + d] / n;
+}
+
+int device_moments_high(dgv_ctx* ctx, const double* d_f, double out[20]) {
+  const Grid& g = ctx->g;
+  if (!ctx->d_hi && hipMalloc(&ctx->d_hi, sizeof(double) * (kHiBlocks * kHiSums + 32)) != hipSuccess) {
+    ctx->d_hi = nullptr;
+    return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(moment scratch) failed");
+  }
+  double* part = ctx->d_red;
+  double* mean = ctx->d_red + kRedBlocks * 5;
+  double* res = ctx->d_hi + kHiBlocks * kHiSums;
+  hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 0, g.N, d_f, ctx->d_gwts,
+                     ctx->d_u, ctx->d_v, ctx->d_w, mean, part);
+  hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(1), 0, ctx->stream, 0, part, mean);
+  hipLaunchKernelGGL(k_high_part, dim3(kHiBlocks), dim3(256), 0, ctx->stream, g.N, d_f, ctx->d_gwts, ctx->d_u,
+                     ctx->d_v, ctx->d_w, mean, ctx->d_hi);
+  hipLaunchKernelGGL(k_high_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_hi, mean, res);
+  DGV_HIP(ctx, hipGetLastError());
+  DGV_HIP(ctx, hipMemcpyAsync(out, res, 20 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
 // Projection onto the complement of span{1,u,v,w,|u|^2} in the w-weighted
 // inner product: lambda = G^{-1} b with G_jk = sum w psi_j psi_k (Cholesky
 // factor precomputed on the host in extended precision), q -= sum lambda_k psi_k.

@@ -57,6 +57,7 @@ module dgv_gpu
   public :: EvalCollisionPeriodicAPlus_DGV, EvalCollisionPeriodicA_DGV, SpatialOperSH_DGV
   public :: TimeIntegratorORK_SH_DGV, TimeIntegratorMTS_SH_DGV, PrepareMTS_SH_DGV, MassCheckRec
   public :: EvalCollisionPeriodicMixedTermsA_DGV, PrepareFMA_DGV, EvalCollisionLinear, CheckSolutionMode_DGV
+  public :: MassCheckRecHighCMoments, dgv_gpu_get_history, dgv_gpu_set_history
 
   interface
     integer(c_int) function c_ctx_create(grid, device, out) bind(C, name="dgv_ctx_create")
@@ -183,6 +184,13 @@ module dgv_gpu
       type(c_ptr), value :: c
       real(c_double), intent(out) :: g(*)
       integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function c_moments_high(c, f, n, out) bind(C, name="dgv_moments_high")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*)
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: out(20)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
@@ -377,6 +385,48 @@ contains
     real(c_double) :: out(5)
     call check(c_moments(ctx, f, size(f, kind=c_int64_t), out), "MassCheckRec")
     n = out(1); ubar = out(2); vbar = out(3); wbar = out(4); temp = out(5)
+  end subroutine
+
+  subroutine MassCheckRecHighCMoments(f, n, ubar, vbar, wbar, temp, temp_u, temp_v, temp_w, mom3_u, mom3_v, &
+                                      mom3_w, mom4_u, mom4_v, mom4_w, mom5_u, mom5_v, mom5_w, mom6_u, mom6_v, mom6_w)
+    real(DP), dimension(:), intent(in) :: f
+    real(DP), intent(out) :: n, ubar, vbar, wbar, temp, temp_u, temp_v, temp_w, mom3_u, mom3_v, mom3_w
+    real(DP), intent(out) :: mom4_u, mom4_v, mom4_w, mom5_u, mom5_v, mom5_w, mom6_u, mom6_v, mom6_w
+    real(c_double) :: o(20)
+    call check(c_moments_high(ctx, f, size(f, kind=c_int64_t), o), "MassCheckRecHighCMoments")
+    n = o(1); ubar = o(2); vbar = o(3); wbar = o(4); temp = o(5)
+    temp_u = o(6); temp_v = o(7); temp_w = o(8)
+    mom3_u = o(9); mom3_v = o(10); mom3_w = o(11); mom4_u = o(12); mom4_v = o(13); mom4_w = o(14)
+    mom5_u = o(15); mom5_v = o(16); mom5_w = o(17); mom6_u = o(18); mom6_v = o(19); mom6_w = o(20)
+  end subroutine
+
+  ! The multistep history the steppers keep (commvar f1..f4, frhs1..frhs5 in the
+  ! reference): copy it out before the reference's WriteSol_DGV and back in
+  ! after its RestartSolutionDGV.
+  subroutine dgv_gpu_get_history(f1, f2, f3, f4, rhs1, rhs2, rhs3, rhs4, rhs5)
+    real(DP), dimension(:), intent(out), optional :: f1, f2, f3, f4, rhs1, rhs2, rhs3, rhs4, rhs5
+    if (present(f1)) f1 = hist(:, 1)
+    if (present(f2)) f2 = hist(:, 2)
+    if (present(f3)) f3 = hist(:, 3)
+    if (present(f4)) f4 = hist(:, 4)
+    if (present(rhs1)) rhs1 = frhs1
+    if (present(rhs2)) rhs2 = hist(:, 6)
+    if (present(rhs3)) rhs3 = hist(:, 7)
+    if (present(rhs4)) rhs4 = hist(:, 8)
+    if (present(rhs5)) rhs5 = hist(:, 9)
+  end subroutine
+
+  subroutine dgv_gpu_set_history(f1, f2, f3, f4, rhs1, rhs2, rhs3, rhs4, rhs5)
+    real(DP), dimension(:), intent(in), optional :: f1, f2, f3, f4, rhs1, rhs2, rhs3, rhs4, rhs5
+    if (present(f1)) hist(:, 1) = f1
+    if (present(f2)) hist(:, 2) = f2
+    if (present(f3)) hist(:, 3) = f3
+    if (present(f4)) hist(:, 4) = f4
+    if (present(rhs1)) frhs1 = rhs1
+    if (present(rhs2)) hist(:, 6) = rhs2
+    if (present(rhs3)) hist(:, 7) = rhs3
+    if (present(rhs4)) hist(:, 8) = rhs4
+    if (present(rhs5)) hist(:, 9) = rhs5
   end subroutine
 
 end module dgv_gpu

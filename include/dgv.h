@@ -207,6 +207,30 @@ int dgv_ab_step(dgv_ctx* ctx, double* f, int64_t n, double dt, double coef, int3
 
 /* MassCheckRec (dgvtools_mod.f90:1437-1458): out = {n, ubar, vbar, wbar, T} */
 int dgv_moments(dgv_ctx* ctx, const double* f, int64_t n, double out[5]);
+/* MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549): out = {n, ubar, vbar,
+ * wbar, T, T_u, T_v, T_w, mom3_u, mom3_v, mom3_w, mom4_*, mom5_*, mom6_*} with
+ * mom_k_d = sum f w (u_d - ubar_d)^k / n and T_d = sum f w (u_d - ubar_d)^2 / n * 2/3 */
+int dgv_moments_high(dgv_ctx* ctx, const double* f, int64_t n, double out[20]);
+int dgv_moments_high_dev(dgv_ctx* ctx, const double* d_f, int64_t n, double out[20]);
+
+/* ---- solution checkpoint and driver files (host only) ----------------------------- */
+/* MakeBaseNameDGV (readwrite.f90:1505-1560, uniform meshes) + "_time" + trim(adjustl(suff))
+ * + "_sol.dat", the name WriteSol_DGV / RestartSolutionDGV use */
+int dgv_sol_file_name(const dgv_grid_desc* grid, const char* dir, const char* base, int32_t k_c,
+                      int32_t N_x, const char* suff, char* out, size_t cap);
+/* MakeBaseNameDGV + suffix ("_macroerr.txt" of WriteErrHighCMoments_DGV,
+ * "_L1err.txt" of KeepTrackL1_err_DGV) */
+int dgv_log_file_name(const dgv_grid_desc* grid, const char* dir, const char* base, int32_t k_c,
+                      int32_t N_x, const char* suffix, char* out, size_t cap);
+/* WriteSol_DGV (readwrite.f90:1756-1796): Fortran-unformatted records
+ * (e_time, dt), (rkmts), (size(f)), (f, f1..f_{k-1}, frhs1..frhs_{k-1});
+ * hist = f1..f4, frhs1..frhs5 (9 n doubles, may be NULL for rkmts 1) */
+int dgv_sol_write_file(const char* path, double e_time, double dt, int32_t rkmts, int64_t n,
+                       const double* f, const double* hist);
+/* RestartSolutionDGV (readwrite.f90:2389-2513): the reverse; fails when the
+ * stored size differs from n.  rkmts receives the file's order. */
+int dgv_sol_read_file(const char* path, double* e_time, double* dt, int32_t* rkmts, int64_t n,
+                      double* f, double* hist);
 /* sum_i w_i psi_k(v_i) q_i for psi = 1,u,v,w,|u|^2 (RecordMomDerivRelaxTimes_DGV's
  * conservation check, dgvtools_mod.f90:1680-1700), FP64 on the device */
 int dgv_collision_moments(dgv_ctx* ctx, const double* q, int64_t n, double out[5]);

@@ -17,6 +17,7 @@ function               reference routine restated
 ``eval_aplus_comp``    the same loop with Neumaier-compensated accumulation
 ``iphi``               IphiDGV (dgvtools_mod.f90:847-877)
 ``masscheck``          MassCheckRec (dgvtools_mod.f90:1437-1458)
+``masscheck_high``     MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549)
 ``maxwell``            maxwelveldist (distributions_mod.f90:83-94)
 ``rk_step``            SpatialOperSH_DGV run_mode 0 + TimeIntegratorORK_SH_DGV
                        (spat_oper_mod.f90:28-68, time_integr_mod.f90:26-152)
@@ -111,6 +112,7 @@ def lib():
         L.orc_iphi.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p, _f64p, _f64p,
                                C.c_double, _f64p]
         L.orc_masscheck.argtypes = [C.POINTER(_Geom), _f64p, _f64p]
+        L.orc_masscheck_high.argtypes = [C.POINTER(_Geom), _f64p, _f64p]
         L.orc_maxwell.argtypes = [C.c_double] * 8
         L.orc_maxwell.restype = C.c_double
         L.orc_spatial_oper.argtypes = [_QFN, C.c_void_p, C.c_int64, _f64p, _f64p, C.c_double,
@@ -254,6 +256,13 @@ def iphi(capphi, A, A_xi, A_xi1, gwts, f, coef=1.0):
 def masscheck(G: Geometry, f):
     out = np.zeros(5)
     lib().orc_masscheck(G.ptr, _c(f, np.float64), out)
+    return out
+
+
+def masscheck_high(G: Geometry, f):
+    """MassCheckRecHighCMoments: n, ubar, vbar, wbar, T, Tu, Tv, Tw, mom3..6 (u, v, w)."""
+    out = np.zeros(20)
+    lib().orc_masscheck_high(G.ptr, _c(f, np.float64), out)
     return out
 
 

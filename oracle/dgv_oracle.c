@@ -450,6 +450,36 @@ void orc_masscheck(const orc_geom* G, const double* f, double out[5]) {
   out[4] = st / n / 3.0 * 2.0;
 }
 
+void orc_masscheck_high(const orc_geom* G, const double* f, double out[20]) {
+  /* MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549): every sum its own
+   * loop in node order, (x - xbar)**k as repeated products */
+  int64_t nn = G->nnodes;
+  const double *w = G->nodes_gwts, *u = G->nodes_u, *v = G->nodes_v, *ww = G->nodes_w;
+  double base[5];
+  orc_masscheck(G, f, base);
+  const double n = base[0], mean[3] = {base[1], base[2], base[3]};
+  const double* x[3] = {u, v, ww};
+  for (int k = 0; k < 5; k++) out[k] = base[k];
+  for (int d = 0; d < 3; d++) {
+    double s = 0;
+    for (int64_t i = 0; i < nn; i++) {
+      double t = x[d][i] - mean[d];
+      s = s + f[i] * w[i] * (t * t);
+    }
+    out[5 + d] = s / n / 3.0 * 2.0;
+  }
+  for (int k = 3; k <= 6; k++)
+    for (int d = 0; d < 3; d++) {
+      double s = 0;
+      for (int64_t i = 0; i < nn; i++) {
+        double t = x[d][i] - mean[d], p = t;
+        for (int j = 1; j < k; j++) p = p * t;
+        s = s + f[i] * w[i] * p;
+      }
+      out[8 + 3 * (k - 3) + d] = s / n;
+    }
+}
+
 double orc_maxwell(double T, double u0, double v0, double w0, double n, double u, double v,
                    double w) {
   double beta = sqrt(PI25DT * T) * (PI25DT * T);

@@ -75,6 +75,7 @@ int orc_iphi(int64_t nnodes, const int32_t* capphi, const double* A,
 
 /* a6: MassCheckRec (dgvtools_mod.f90:1437-1458): out = {n, ubar, vbar, wbar, T} */
 void orc_masscheck(const orc_geom* G, const double* f, double out[5]);
+void orc_masscheck_high(const orc_geom* G, const double* f, double out[20]);
 /* maxwelveldist (distributions_mod.f90:83-94) */
 double orc_maxwell(double T, double u0, double v0, double w0, double n,
                    double u, double v, double w);

@@ -1,0 +1,133 @@
+"""The driver's files and moments, no GPU.
+
+ * MassCheckRecHighCMoments (oracle): on a Maxwellian exp(-|u - ubar|^2 / T)
+   sampled on a fine grid the closed forms hold -- T_d = T/3 (the reference's
+   2/3 factor on a one-dimensional variance T/2), odd central moments 0,
+   mom4 = 3 (T/2)^2, mom6 = 15 (T/2)^3 -- and T = T_u + T_v + T_w for any f;
+ * the solution checkpoint of WriteSol_DGV / RestartSolutionDGV: the library's
+   file is byte for byte the file a Fortran program writes with the
+   reference's write statements (built with amdflang here), reads back for
+   every rkmts, and a size mismatch is refused;
+ * the file names of MakeBaseNameDGV, and the macro / L1 log lines against the
+   same Fortran runtime's formatting of the reference's format strings.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import bimodal
+
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+needs_fc = pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not installed")
+
+
+def test_high_moments_closed_forms_on_a_maxwellian():
+    G = oracle.Geometry(7, 6, (-6.0, 6.0))
+    T, ub = 1.3, (0.2, -0.1, 0.05)
+    f = np.array([oracle.maxwell(T, *ub, 1.0, u, v, w) for u, v, w in zip(G.nodes_u, G.nodes_v, G.nodes_w)])
+    m = oracle.masscheck_high(G, f)
+    s2 = T / 2
+    assert abs(m[0] - 1.0) < 1e-6 and abs(m[4] / T - 1) < 1e-6
+    assert np.allclose(m[1:4], ub, atol=1e-7)
+    assert np.allclose(m[5:8], T / 3, rtol=1e-6)
+    assert np.all(np.abs(m[8:11]) < 1e-6) and np.all(np.abs(m[14:17]) < 1e-6)
+    assert np.allclose(m[11:14], 3 * s2 ** 2, rtol=1e-6)
+    assert np.allclose(m[17:20], 15 * s2 ** 3, rtol=1e-6)
+    # equals MassCheckRec on the common five
+    assert np.array_equal(m[:5], oracle.masscheck(G, f))
+
+
+def test_high_moments_temperature_splits_by_direction():
+    G = oracle.Geometry(4, 3, (-3.0, 3.0))
+    f = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, 2)
+    m = oracle.masscheck_high(G, f)
+    assert abs(m[4] - (m[5] + m[6] + m[7])) <= 1e-14 * m[4]
+
+
+FORTRAN_SOL = """
+program w
+  implicit none
+  integer, parameter :: DP = kind(1.0d0)
+  integer :: rkmts, n, i
+  real(DP) :: e_time, dt
+  real(DP), allocatable :: f(:), f1(:), f2(:), frhs1(:), frhs2(:)
+  n = 7; rkmts = 3; e_time = 0.125_DP; dt = 0.0025_DP
+  allocate(f(n), f1(n), f2(n), frhs1(n), frhs2(n))
+  do i = 1, n
+    f(i) = 1.0_DP / i; f1(i) = 2.0_DP / i; f2(i) = 3.0_DP / i
+    frhs1(i) = -1.0_DP * i; frhs2(i) = -2.0_DP * i
+  end do
+  open (15, file="ref_sol.dat", position="REWIND", action="WRITE", form="UNFORMATTED", access="SEQUENTIAL")
+  write (15) e_time, dt
+  write (15) rkmts
+  write (15) size(f, 1)
+  write (15) f, f1, f2, frhs1, frhs2
+  close (15)
+  open (16, file="macro.txt", form="FORMATTED")
+  write (16, "(F18.14, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15,  A2, F18.15, " // &
+            "A2, F18.14, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2," // &
+            "F18.14, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15 )") &
+    0.5_DP, ",", 1.0_DP, ",", 0.1_DP, ",", -0.2_DP, ",", 0.3_DP, ",", 0.75_DP, ",", 0.25_DP, ",", 0.25_DP, ",", &
+    0.25_DP, ",", -0.001_DP, ",", 0.002_DP, ",", 0.0_DP, ",", 0.1875_DP, ",", 0.19_DP, ",", 0.2_DP, ",", &
+    -1.0e-5_DP, ",", 3.0e-4_DP, ",", 0.0_DP, ",", 0.25_DP, ",", 12.5_DP, ",", -12.5_DP
+  write (16, "(F18.14, A2, F18.15)") 0.0125_DP, ",", 0.0421_DP
+  close (16)
+end program w
+"""
+
+
+@needs_fc
+def test_solution_file_matches_fortran_and_reads_back(tmp_path):
+    import dgv0d3v_direct_collision_amd as dgv
+    (tmp_path / "w.f90").write_text(FORTRAN_SOL)
+    subprocess.run([FC, "-O0", "w.f90", "-o", "w"], cwd=tmp_path, check=True)
+    subprocess.run(["./w"], cwd=tmp_path, check=True)
+    n = 7
+    i = np.arange(1, n + 1, dtype=np.float64)
+    f = 1.0 / i
+    hist = np.zeros((9, n))
+    hist[0], hist[1] = 2.0 / i, 3.0 / i
+    hist[4], hist[5] = -1.0 * i, -2.0 * i
+    ours = tmp_path / "ours.dat"
+    dgv.write_sol_file(ours, 0.125, 0.0025, 3, f, hist)
+    assert ours.read_bytes() == (tmp_path / "ref_sol.dat").read_bytes()
+    e, dt, k, f2, h2 = dgv.read_sol_file(tmp_path / "ref_sol.dat", n)
+    assert (e, dt, k) == (0.125, 0.0025, 3)
+    assert np.array_equal(f2, f) and np.array_equal(h2[[0, 1, 4, 5]], hist[[0, 1, 4, 5]])
+    with pytest.raises(dgv.DGVError):
+        dgv.read_sol_file(tmp_path / "ref_sol.dat", n + 1)
+    # the log lines against the Fortran runtime's formatting
+    lines = (tmp_path / "macro.txt").read_text().splitlines()
+    vals = [1.0, 0.1, -0.2, 0.3, 0.75, 0.25, 0.25, 0.25, -0.001, 0.002, 0.0, 0.1875, 0.19, 0.2, -1.0e-5, 3.0e-4,
+            0.0, 0.25, 12.5, -12.5]
+    from dgv0d3v_direct_collision_amd.driver_io import fortran_f, macro_line
+    assert macro_line(0.5, vals) == lines[0]
+    assert fortran_f(0.0125, 18, 14) + " ," + fortran_f(0.0421, 18, 15) == lines[1]
+
+
+@pytest.mark.parametrize("rkmts", [1, 2, 3, 4, 5])
+def test_solution_file_round_trip(tmp_path, rkmts):
+    import dgv0d3v_direct_collision_amd as dgv
+    rng = np.random.default_rng(rkmts)
+    n = 31
+    f = rng.standard_normal(n)
+    hist = rng.standard_normal((9, n))
+    p = tmp_path / "s.dat"
+    dgv.write_sol_file(p, 1.5, 0.01, rkmts, f, hist)
+    assert os.path.getsize(p) == (8 + 16) + (8 + 4) * 2 + 8 + 8 * n * (2 * rkmts - 1)
+    e, dt, k, f2, h2 = dgv.read_sol_file(p, n)
+    assert (e, dt, k) == (1.5, 0.01, rkmts) and np.array_equal(f2, f)
+    rows = list(range(rkmts - 1)) + list(range(4, 4 + rkmts - 1))
+    assert np.array_equal(h2[rows], hist[rows])
+
+
+def test_file_names():
+    import dgv0d3v_direct_collision_amd as dgv
+    assert dgv.sol_file_name(9, 3, (-4.0, 4.0), " 0.0100000000", "sol/", "run", k_c=2, N_x=3) == \
+        "sol/run2kc3su3sv3sw3NXU9MuUU9MvVU9MwWU_time0.0100000000_sol.dat"
+    assert dgv.log_file_name(5, 2, (-1.0, 1.0), "_L1err.txt", "", "b", k_c=1, N_x=1) == \
+        "b1kc2su2sv2sw1NXU5MuUU5MvVU5MwWU_L1err.txt"

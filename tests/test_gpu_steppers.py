@@ -124,3 +124,11 @@ def test_projected_rk_graph_conserves_moments(env):
     scale = np.abs(f * G.nodes_gwts) @ np.abs(psi.T)
     assert np.all(np.abs(m1 - m0) <= 1e-13 * scale), m1 - m0
     assert rel_l2(fr, f) > 1e-6            # the VDF did change
+
+
+def test_high_moments_match_oracle(env):
+    G, A, d, f, q = env
+    ref = oracle.masscheck_high(G, f)
+    got = np.array(d.MassCheckRecHighCMoments(f))
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(np.abs(ref), 1e-3))
+    assert np.allclose(got[:5], d.MassCheckRec(f), rtol=1e-14, atol=0)
