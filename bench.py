@@ -579,27 +579,27 @@ def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
     n = args.sur_rows
     nte = max(32, n // 8)
     ntr = n - nte
-    log(f"surrogate: generating {n} (f, Q) pairs with the batched FP32-compensated evaluator")
-    fs = np.stack([mixture_vdf(u, v, w, gw, rng, kmax=4) if i % 2 else config_vdf(u, v, w, gw, M, rng)
-                   for i in range(n)])
-    qs = np.zeros_like(fs)
+    log(f"surrogate: writing {n} (f, Q) pairs with the training-data writer (batched FP32-compensated)")
+    import tempfile
+    from dgv0d3v_direct_collision_amd import dataset
     tg = time.perf_counter()
+    with tempfile.TemporaryDirectory(prefix="dgv_train_") as tdir:
+        meta = dataset.TrainingSetWriter(tdir, d, n, precision=dgv.DGV_FP32_COMP, batch=256, seed_base=20261207,
+                                         tile=args.tile if args.tile > 0 else 262144).run()
+        gen_s = time.perf_counter() - tg
+        check = dataset.verify(tdir, d, nsample=4)
+        _, fmap, qmap, _ = dataset.load(tdir)
+        fs, qs = np.array(fmap), np.array(qmap)
+        del fmap, qmap
+    writer = {"rows": n, "seconds": gen_s, "rows_per_s": n / gen_s, "gpu_seconds": meta["gpu_seconds"],
+              "mb_per_s": 2 * fs.nbytes / gen_s / 1e6, "verify": check}
+    log(f"surrogate: {gen_s:.1f} s for the pairs ({n / gen_s:.0f} rows/s, GPU {meta['gpu_seconds']:.1f} s); "
+        f"verify {check}; creating the network")
     d_f = d.device_alloc(fs.nbytes)
     d_q = d.device_alloc(fs.nbytes)
     d_p = d.device_alloc(fs.nbytes)
     d.htod(d_f, fs)
-    chunk = min(256, n)
-    b = dgv.DGVBatch([d], np.zeros(chunk, np.int32), dgv.DGV_FP32_COMP)
-    b.set_tile(args.tile if args.tile > 0 else 262144)
-    for r0 in range(0, n, chunk):
-        k = min(chunk, n - r0)
-        b.set_f_rows(d_f + 8 * N * r0, 0, k)
-        b.eval_q()
-        b.get_q_rows(d_q + 8 * N * r0, 0, k)
-    b.close()
-    d.dtoh(qs, d_q)
-    gen_s = time.perf_counter() - tg
-    log(f"surrogate: {gen_s:.1f} s for the pairs; creating the network")
+    d.htod(d_q, qs)
     s = dgv.Surrogate(d, [N, 2048, 2048, 2048, N], batch=256, seed=11)
     s.set_scales(float(np.sqrt(np.mean(fs[:ntr] ** 2))), float(np.sqrt(np.mean(qs[:ntr] ** 2))))
     tt = time.perf_counter()
@@ -628,7 +628,8 @@ def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
     for ptr in (d_f, d_q, d_p):
         d.device_free(ptr)
     out = {"widths": [N, 2048, 2048, 2048, N], "train_rows": ntr, "test_rows": nte, "steps": args.sur_steps,
-           "batch": 256, "lr": 1e-4, "data_gen_seconds": gen_s, "train_seconds": train_s,
+           "batch": 256, "lr": 1e-4, "data_gen_seconds": gen_s, "training_set_writer": writer,
+           "train_seconds": train_s,
            "train_steps_per_s": args.sur_steps / train_s, "loss_first": losses[0],
            "loss_last20": float(np.mean(losses[-20:])),
            "inference_vdf_per_s": nte / (ms_inf * 1e-3), "inference_ms_per_vdf": ms_inf / nte,
