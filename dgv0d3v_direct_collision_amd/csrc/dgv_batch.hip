@@ -162,6 +162,7 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
 // (hi, lo) pair with two-sum.  FP64: waves 2c and 2c+1 share chunk c (32
 // instances each); block sums are added into an FP64 running total.
 constexpr int kWavesPerItem = 4;
+constexpr int kPadPairs = 64;   // zero pairs after the FP32 coefficients (>= 4 kRing + 3)
 
 template <bool FP32>
 constexpr int chunks_per_item() { return FP32 ? 4 : 2; }
@@ -179,6 +180,8 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   constexpr int CT = 4 / WPC;                    // column tiles per wave
   using T = typename std::conditional<FP32, float, double>::type;
   using V4 = typename std::conditional<FP32, bf4, bd4>::type;
+  typedef T VB __attribute__((ext_vector_type(CT)));
+  static_assert(sizeof(VB) == 16, "one 16-byte LDS read per lane and step");
   __shared__ T lfa[CH][D * 64], lfb[CH][D * 64];
   __shared__ int s_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -186,6 +189,14 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   const int ch = wave / WPC;                     // this wave's chunk within the item
   const int sub = wave % WPC;                    // its part of the chunk's staging / columns
   const int col0 = sub * 16 * CT;
+  const int lcol = col0 + CT * jj;               // the lane's first instance in a staged row
+  // FP32: the lane's byte offsets inside a step of 4 pairs (coefficient rows
+  // 16 rt + jj, clamped to the code slot D, and the pair's code)
+  uint32_t loffb[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * (int)sizeof(float));
+  const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * (int)sizeof(float));
+  const V4 vzero = V4{0, 0, 0, 0};
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
@@ -211,6 +222,19 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
     T* la = lfa[ch];
     T* lb = lfb[ch];
     int prev_xc = -1;   // the cell whose f is in la (blocks come ordered by a)
+    // fold a round's matrix-core sums into the FP32 (hi, lo) pair (the next
+    // round's first products start a fresh sum, so acc needs no clearing)
+#define FOLD()                                                                                \
+  do {                                                                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                         \
+    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                           \
+      float s2_, er_;                                                                         \
+      two_sum(hi[rt][ct][e], acc[rt][ct][e], s2_, er_);                                       \
+      hi[rt][ct][e] = s2_;                                                                    \
+      lo[rt][ct][e] += er_;                                                                   \
+    }                                                                                         \
+  } while (0)
     for (int bi = it.b0; bi < it.b1; ++bi) {
       const BlockHdr* B = G.blocks + G.blist[bi];
       const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
@@ -237,82 +261,139 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       __syncthreads();
       const int n = B->npairs;
       if (!active || n == 0) continue;
-      // Coefficient operand of step s (4 pairs): lane i holds T[pair s + i/16][16 rt + i%16].
-      // A ring of kRing steps is kept in flight; the loads are unconditional
-      // global loads (indices clamped into the block's records; rows >= D read
-      // the record's code slot and only feed output rows that are dropped), and
-      // steps past the block's end carry code -1, so their products are 0.
       constexpr int kRing = 8;
-      const AS1 T* P;
-      if constexpr (FP32) P = (const AS1 T*)(G.pairs32 + B->pbeg * (D + 1));
-      else P = (const AS1 T*)(G.pairs64 + B->pbeg * (D + 1));
       T ab[kRing][RT];
       int cbits[kRing];
-      bool cok[kRing];
-#define BATCH_LOAD(slot, s)                                                                   \
+      VB ra, rb;
+      T bcur[CT];
+      // the B operand of a step: LDS reads issued one step ahead of its products.
+      // Column jj of tile ct is instance col0 + CT jj + ct, so a lane's CT
+      // values are one 16-byte LDS read and the four pairs of a step (four
+      // lane groups, different rows q) read disjoint 16-byte runs.
+#define B_ISSUE(c)                                                                            \
+  do {                                                                                        \
+    const int c_ = (c);                                                                       \
+    ra = *reinterpret_cast<const VB*>(la + (c_ >> 8) * 64 + lcol);                            \
+    rb = *reinterpret_cast<const VB*>(lb + (c_ & 255) * 64 + lcol);                           \
+  } while (0)
+#define STEP_MFMA(u, fresh)                                                                   \
+  do {                                                                                        \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                         \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                       \
+      if constexpr (FP32)                                                                     \
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[u][rt], bcur[ct],               \
+                                                           (fresh) ? vzero : acc[rt][ct], 0, 0, 0); \
+      else                                                                                    \
+        acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0); \
+    }                                                                                         \
+  } while (0)
+      if constexpr (FP32) {
+        // Coefficient operand of step s (4 pairs): lane i holds T[pair 4s + i/16][16 rt + i%16]
+        // (rows >= D read the pair's code slot and only feed output rows that are
+        // dropped).  The FP32 copy of the coefficients carries zero pairs past its
+        // end, so the ring of kRing steps in flight reads straight ahead -- a
+        // uniform step base plus the lane's fixed byte offset, no clamping; only
+        // the block's last, partial step masks the pairs past its end.
+        const AS1 char* P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 1));
+        constexpr int64_t kStepBytes = 4 * (D + 1) * (int64_t)sizeof(float);
+#define LOAD32(slot, step)                                                                    \
+  do {                                                                                        \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+      ab[slot][rt] = *(const AS1 float*)(Ps_ + loffb[rt]);                                    \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                             \
+  } while (0)
+#define FINISH32()                                                                            \
+  do {                                                                                        \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = ra[ct] * rb[ct];             \
+  } while (0)
+        const int nfull = n >> 2, nst = (n + 3) >> 2;
+#pragma unroll
+        for (int u = 0; u < kRing; ++u) LOAD32(u, u);
+        B_ISSUE(cbits[0]);
+        FINISH32();
+        int st = 0;
+        for (; st + kRing <= nfull; st += kRing) {   // full rounds: the ring refills kRing steps ahead
+#pragma unroll
+          for (int u = 0; u < kRing; ++u) {
+            B_ISSUE(cbits[(u + 1) % kRing]);   // next step (slot 0 already holds the next round's first)
+            STEP_MFMA(u, u == 0);               // each round starts a fresh matrix-core sum
+            LOAD32(u, st + u + kRing);
+            FINISH32();
+          }
+          FOLD();
+        }
+        if (st < nst) {   // the last nst - st <= kRing steps, already in the ring
+#pragma unroll
+          for (int u = 0; u < kRing; ++u) {
+            if (st + u < nst) {
+              if (st + u == nfull) {   // partial step: pairs past the block's end contribute 0
+                const bool ok = 4 * nfull + kk < n;
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) ab[u][rt] = ok ? ab[u][rt] : 0.0f;
+              }
+              B_ISSUE(cbits[(u + 1) % kRing]);
+              STEP_MFMA(u, u == 0);
+              FINISH32();
+            }
+          }
+          FOLD();
+        }
+#undef FINISH32
+#undef LOAD32
+      } else {
+        // FP64: the coefficients are the context's own array (no padding), so the
+        // ring's indices are clamped into the block and steps past its end carry
+        // a zero B operand.
+        const AS1 T* P = (const AS1 T*)(G.pairs64 + B->pbeg * (D + 1));
+        bool cok[kRing];
+        bool bok;
+#define LOAD64(slot, s)                                                                       \
   do {                                                                                        \
     const int pv_ = (s) + kk;                                                                 \
     const AS1 T* R_ = P + (int64_t)min(pv_, n - 1) * (D + 1);                                 \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = R_[min(16 * rt + jj, D)]; \
-    if constexpr (FP32) cbits[slot] = __float_as_int(R_[D]);                                  \
-    else cbits[slot] = (int)__double_as_longlong(R_[D]);                                      \
+    cbits[slot] = (int)__double_as_longlong(R_[D]);                                           \
     cok[slot] = pv_ < n;                                                                      \
   } while (0)
-      // the B operand of a step: LDS reads issued one step ahead of its products
-      T ra[CT], rb[CT], bcur[CT];
-      bool bok;
-#define B_ISSUE(slot)                                                                         \
+#define ISSUE64(slot)                                                                         \
   do {                                                                                        \
     bok = cok[slot];                                                                          \
-    const int cz_ = bok ? cbits[slot] : 0;                                                    \
-    const int p_ = cz_ >> 8, q_ = cz_ & 255;                                                  \
-    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                       \
-      ra[ct] = la[p_ * 64 + col0 + 16 * ct + jj];                                             \
-      rb[ct] = lb[q_ * 64 + col0 + 16 * ct + jj];                                             \
-    }                                                                                         \
+    B_ISSUE(bok ? cbits[slot] : 0);                                                           \
   } while (0)
-#define B_FINISH()                                                                            \
+#define FINISH64()                                                                            \
   do {                                                                                        \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = bok ? ra[ct] * rb[ct] : (T)0; \
   } while (0)
+        const int nst = (n + 3) >> 2;
 #pragma unroll
-      for (int u = 0; u < kRing; ++u) BATCH_LOAD(u, 4 * u);
-      B_ISSUE(0);
-      B_FINISH();
-      for (int s0 = 0; s0 < n; s0 += 4 * kRing) {
+        for (int u = 0; u < kRing; ++u) LOAD64(u, 4 * u);
+        ISSUE64(0);
+        FINISH64();
+        int st = 0;
+        for (; st + kRing <= nst; st += kRing) {
+#pragma unroll
+          for (int u = 0; u < kRing; ++u) {
+            ISSUE64((u + 1) % kRing);
+            STEP_MFMA(u, false);
+            LOAD64(u, 4 * (st + u + kRing));
+            FINISH64();
+          }
+        }
 #pragma unroll
         for (int u = 0; u < kRing; ++u) {
-          B_ISSUE((u + 1) % kRing);   // next step (slot 0 already holds the next round's first)
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-              if constexpr (FP32)
-                acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
-              else
-                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
-            }
-          BATCH_LOAD(u, s0 + 4 * u + 4 * kRing);
-          B_FINISH();
+          if (st + u < nst) {
+            ISSUE64((u + 1) % kRing);
+            STEP_MFMA(u, false);
+            FINISH64();
+          }
         }
-        if constexpr (FP32) {  // fold the sums of these 32 pairs into (hi, lo)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                float s2, er;
-                two_sum(hi[rt][ct][e], acc[rt][ct][e], s2, er);
-                hi[rt][ct][e] = s2;
-                lo[rt][ct][e] += er;
-                acc[rt][ct][e] = 0.0f;
-              }
-        }
+#undef FINISH64
+#undef ISSUE64
+#undef LOAD64
       }
-#undef B_FINISH
+#undef STEP_MFMA
 #undef B_ISSUE
-#undef BATCH_LOAD
       if constexpr (!FP32) {  // two-level FP64 sum
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -323,12 +404,13 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
           }
       }
     }
+#undef FOLD
     if (!active) continue;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        const int v = col0 + 16 * ct + jj;
+        const int v = col0 + CT * jj + ct;
         double* out = partial + (it.prow + 64 * ch + v) * D;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -790,11 +872,16 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
       return fail(DGV_ERR_NOMEM, "block lists");
     b->h_blist[g] = blist;
     if (precision == DGV_FP32_COMP) {
-      const int64_t n = pb.npairs * (D + 1);
-      if (hipMalloc(&b->d_pairs32[g], sizeof(float) * std::max<int64_t>(n, 1)))
+      // zero pairs past the end: the kernel's load ring reads up to 4 kRing + 3
+      // pairs beyond a block without clamping
+      const int64_t n = pb.npairs * (D + 1), pad = (int64_t)kPadPairs * (D + 1);
+      if (hipMalloc(&b->d_pairs32[g], sizeof(float) * (n + pad)))
         return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 coefficients)");
-      hipLaunchKernelGGL(k_pairs_to_f32, dim3(stride_blocks(n, 256)), dim3(256), 0, b->stream, pb.npairs, D,
-                         pb.d_pairs, b->d_pairs32[g]);
+      if (n > 0)
+        hipLaunchKernelGGL(k_pairs_to_f32, dim3(stride_blocks(n, 256)), dim3(256), 0, b->stream, pb.npairs, D,
+                           pb.d_pairs, b->d_pairs32[g]);
+      if (hipMemsetAsync(b->d_pairs32[g] + n, 0, sizeof(float) * pad, b->stream))
+        return fail(DGV_ERR_HIP, "hipMemsetAsync(FP32 coefficient padding)");
     }
     gd[g].pairs64 = pb.d_pairs;
     gd[g].pairs32 = b->d_pairs32[g];
