@@ -19,6 +19,12 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
             "-Wno-unused-result", "-Wno-unused-value", "-munsafe-fp-atomics"]
 
 
+# Per-file additions.  The batched kernel folds its matrix-core sums on the VALU
+# every few steps; with the MFMA results in ordinary VGPRs the fold reads them
+# directly (no v_accvgpr_read per element).
+FILE_FLAGS = {"dgv_batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def hipcc() -> str:
     for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if c and os.path.exists(c):
@@ -34,7 +40,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     for s in srcs:
         o = os.path.join(bdir, os.path.basename(s) + ".o")
-        cmd = [hipcc(), *HIPFLAGS, "-x", "hip", "-c", s, "-o", o]
+        cmd = [hipcc(), *HIPFLAGS, *FILE_FLAGS.get(os.path.basename(s), []), "-x", "hip", "-c", s, "-o", o]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

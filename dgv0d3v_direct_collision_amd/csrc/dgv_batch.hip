@@ -33,7 +33,7 @@ namespace dgv {
 
 struct BatchGridDev {
   const double* pairs64;   // npairs * (D+1) doubles
-  const float* pairs32;    // npairs * (D+1) floats (FP32-compensated path)
+  const float* pairs32;    // npairs * (D+2) floats (FP32-compensated path): T, p and q row offsets
   const BlockHdr* blocks;
   const int32_t* blist;    // per-cell block lists, concatenated
   const double* gwts;
@@ -134,12 +134,22 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) b[i] = (float)a[i];
 }
 
+// FP32 copy of the pair records for the batched kernel: D coefficients, then
+// the byte offsets p * 64 * 4 and q * 64 * 4 of the pair's rows in the staged
+// f columns (64 floats per row), so that the kernel forms its LDS addresses
+// with one add each.
 __global__ void k_pairs_to_f32(int64_t npairs, int D, const double* __restrict__ a, float* __restrict__ b) {
-  const int64_t n = npairs * (D + 1), step = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n = npairs * (D + 2), step = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-    const int l = (int)(i % (D + 1));
-    if (l < D) b[i] = (float)a[i];
-    else b[i] = __int_as_float((int)__double_as_longlong(a[i]));  // pair code
+    const int64_t k = i / (D + 2);
+    const int l = (int)(i - k * (D + 2));
+    const double* r = a + k * (D + 1);
+    if (l < D) {
+      b[i] = (float)r[l];
+    } else {
+      const int code = (int)__double_as_longlong(r[D]);
+      b[i] = __int_as_float(l == D ? (code >> 8) * 256 : (code & 255) * 256);
+    }
   }
 }
 
@@ -194,8 +204,8 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   // 16 rt + jj, clamped to the code slot D, and the pair's code)
   uint32_t loffb[RT];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * (int)sizeof(float));
-  const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * (int)sizeof(float));
+  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 2) + min(16 * rt + jj, D)) * (int)sizeof(float));
+  const uint32_t coffb = (uint32_t)((kk * (D + 2) + D) * (int)sizeof(float));
   const V4 vzero = V4{0, 0, 0, 0};
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -266,15 +276,16 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       int cbits[kRing];
       VB ra, rb;
       T bcur[CT];
+      const char* la_lane = reinterpret_cast<const char*>(la + lcol);
+      const char* lb_lane = reinterpret_cast<const char*>(lb + lcol);
       // the B operand of a step: LDS reads issued one step ahead of its products.
       // Column jj of tile ct is instance col0 + CT jj + ct, so a lane's CT
       // values are one 16-byte LDS read and the four pairs of a step (four
       // lane groups, different rows q) read disjoint 16-byte runs.
-#define B_ISSUE(c)                                                                            \
+#define B_ISSUE_BYTES(pb, qb)                                                                 \
   do {                                                                                        \
-    const int c_ = (c);                                                                       \
-    ra = *reinterpret_cast<const VB*>(la + (c_ >> 8) * 64 + lcol);                            \
-    rb = *reinterpret_cast<const VB*>(lb + (c_ & 255) * 64 + lcol);                           \
+    ra = *reinterpret_cast<const VB*>(la_lane + (pb));                                        \
+    rb = *reinterpret_cast<const VB*>(lb_lane + (qb));                                        \
   } while (0)
 #define STEP_MFMA(u, fresh)                                                                   \
   do {                                                                                        \
@@ -294,14 +305,16 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         // end, so the ring of kRing steps in flight reads straight ahead -- a
         // uniform step base plus the lane's fixed byte offset, no clamping; only
         // the block's last, partial step masks the pairs past its end.
-        const AS1 char* P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 1));
-        constexpr int64_t kStepBytes = 4 * (D + 1) * (int64_t)sizeof(float);
+        const AS1 char* P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 2));
+        constexpr int64_t kStepBytes = 4 * (D + 2) * (int64_t)sizeof(float);
+        int qbits[kRing];
 #define LOAD32(slot, step)                                                                    \
   do {                                                                                        \
     const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
       ab[slot][rt] = *(const AS1 float*)(Ps_ + loffb[rt]);                                    \
     cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                             \
+    qbits[slot] = *(const AS1 int*)(Ps_ + coffb + 4);                                         \
   } while (0)
 #define FINISH32()                                                                            \
   do {                                                                                        \
@@ -310,19 +323,31 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         const int nfull = n >> 2, nst = (n + 3) >> 2;
 #pragma unroll
         for (int u = 0; u < kRing; ++u) LOAD32(u, u);
-        B_ISSUE(cbits[0]);
+        B_ISSUE_BYTES(cbits[0], qbits[0]);
         FINISH32();
+#define ROUND32(st_)                                                                           \
+  do {                                                                                        \
+    _Pragma("unroll") for (int u = 0; u < kRing; ++u) {                                       \
+      B_ISSUE_BYTES(cbits[(u + 1) % kRing], qbits[(u + 1) % kRing]);   /* next step */         \
+      STEP_MFMA(u, u == 0);   /* each round starts a fresh matrix-core sum */                 \
+      LOAD32(u, (st_) + u + kRing);                                                           \
+      FINISH32();                                                                             \
+    }                                                                                         \
+    FOLD();                                                                                   \
+  } while (0)
+        // full rounds, two per iteration (the folds' results alternate between
+        // two register sets instead of being copied back); the ring refills
+        // kRing steps ahead
         int st = 0;
-        for (; st + kRing <= nfull; st += kRing) {   // full rounds: the ring refills kRing steps ahead
-#pragma unroll
-          for (int u = 0; u < kRing; ++u) {
-            B_ISSUE(cbits[(u + 1) % kRing]);   // next step (slot 0 already holds the next round's first)
-            STEP_MFMA(u, u == 0);               // each round starts a fresh matrix-core sum
-            LOAD32(u, st + u + kRing);
-            FINISH32();
-          }
-          FOLD();
+        for (; st + 2 * kRing <= nfull; st += 2 * kRing) {
+          ROUND32(st);
+          ROUND32(st + kRing);
         }
+        if (st + kRing <= nfull) {
+          ROUND32(st);
+          st += kRing;
+        }
+#undef ROUND32
         if (st < nst) {   // the last nst - st <= kRing steps, already in the ring
 #pragma unroll
           for (int u = 0; u < kRing; ++u) {
@@ -332,7 +357,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #pragma unroll
                 for (int rt = 0; rt < RT; ++rt) ab[u][rt] = ok ? ab[u][rt] : 0.0f;
               }
-              B_ISSUE(cbits[(u + 1) % kRing]);
+              B_ISSUE_BYTES(cbits[(u + 1) % kRing], qbits[(u + 1) % kRing]);
               STEP_MFMA(u, u == 0);
               FINISH32();
             }
@@ -359,7 +384,8 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #define ISSUE64(slot)                                                                         \
   do {                                                                                        \
     bok = cok[slot];                                                                          \
-    B_ISSUE(bok ? cbits[slot] : 0);                                                           \
+    const int c_ = bok ? cbits[slot] : 0;                                                     \
+    B_ISSUE_BYTES((c_ >> 8) * 64 * (int)sizeof(T), (c_ & 255) * 64 * (int)sizeof(T));        \
   } while (0)
 #define FINISH64()                                                                            \
   do {                                                                                        \
@@ -393,7 +419,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #undef LOAD64
       }
 #undef STEP_MFMA
-#undef B_ISSUE
+#undef B_ISSUE_BYTES
       if constexpr (!FP32) {  // two-level FP64 sum
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
@@ -874,7 +900,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     if (precision == DGV_FP32_COMP) {
       // zero pairs past the end: the kernel's load ring reads up to 4 kRing + 3
       // pairs beyond a block without clamping
-      const int64_t n = pb.npairs * (D + 1), pad = (int64_t)kPadPairs * (D + 1);
+      const int64_t n = pb.npairs * (D + 2), pad = (int64_t)kPadPairs * (D + 2);
       if (hipMalloc(&b->d_pairs32[g], sizeof(float) * (n + pad)))
         return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 coefficients)");
       if (n > 0)
