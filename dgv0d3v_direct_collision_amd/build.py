@@ -19,10 +19,11 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
             "-Wno-unused-result", "-Wno-unused-value", "-munsafe-fp-atomics"]
 
 
-# Per-file additions.  The batched kernel folds its matrix-core sums on the VALU
-# every few steps; with the MFMA results in ordinary VGPRs the fold reads them
-# directly (no v_accvgpr_read per element).
-FILE_FLAGS = {"dgv_batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# Per-file additions.  The matrix-core kernels keep their sums in ordinary VGPRs
+# (the batched kernel folds them on the VALU every few steps and reads them
+# without v_accvgpr_read moves).
+FILE_FLAGS = {"dgv_batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+              "dgv_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def hipcc() -> str:

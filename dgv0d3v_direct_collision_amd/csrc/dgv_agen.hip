@@ -291,7 +291,8 @@ static int generate_t(dgv_ctx* ctx, const AgenConsts& C, double min_sens) {
   for (int64_t k = 0; k < np; ++k) { dst[k] = nk; nk += keep[k]; }
   AG_HIP(hipMalloc(&d_dst, sizeof(int64_t) * std::max<int64_t>(np, 1)));
   AG_HIP(hipMemcpy(d_dst, dst.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice));
-  AG_HIP(hipMalloc(&d_pairs, sizeof(double) * (D + 1) * std::max<int64_t>(nk, 1)));
+  AG_HIP(hipMalloc(&d_pairs, sizeof(double) * (D + 1) * (nk + kPairPad)));
+  AG_HIP(hipMemsetAsync(d_pairs + (D + 1) * nk, 0, sizeof(double) * (D + 1) * kPairPad, st));
   if (np > 0)
     hipLaunchKernelGGL(k_compact<D>, dim3(stride_blocks(np, 256)), dim3(256), 0, st, np, d_keep, d_dst, d_raw,
                        d_pairs);

@@ -288,9 +288,12 @@ static int finish_pairblock(dgv_ctx* ctx, std::vector<uint64_t>& pkeys, std::vec
 int upload_pairblock(dgv_ctx* ctx, std::vector<double>& pairs) {
   PairBlock& pb = ctx->pb;
   DGV_HIP(ctx, hipSetDevice(ctx->device));
-  const size_t bp = sizeof(double) * std::max<size_t>(pairs.size(), 1);
-  if (hipMalloc(&pb.d_pairs, bp) != hipSuccess) return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(pairs) failed");
+  const size_t pad = (size_t)kPairPad * (pb.D + 1);
+  if (hipMalloc(&pb.d_pairs, sizeof(double) * (pairs.size() + pad)) != hipSuccess)
+    return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(pairs) failed");
   DGV_HIP(ctx, hipMemcpy(pb.d_pairs, pairs.data(), sizeof(double) * pairs.size(), hipMemcpyHostToDevice));
+  const std::vector<double> zeros(pad, 0.0);
+  DGV_HIP(ctx, hipMemcpy(pb.d_pairs + pairs.size(), zeros.data(), sizeof(double) * pad, hipMemcpyHostToDevice));
   DGV_HIP(ctx, hipMalloc(&pb.d_blocks, sizeof(BlockHdr) * std::max<size_t>(pb.blocks.size(), 1)));
   DGV_HIP(ctx, hipMemcpy(pb.d_blocks, pb.blocks.data(), sizeof(BlockHdr) * pb.blocks.size(), hipMemcpyHostToDevice));
   DGV_HIP(ctx, hipMalloc(&pb.d_group_blocks, sizeof(int32_t) * std::max<size_t>(pb.group_blocks.size(), 1)));

@@ -32,6 +32,9 @@
 namespace dgv {
 
 constexpr int kMaxS = 8;
+// Zero pairs after every pair-record array on the device: the evaluators' load
+// rings read up to 4 * 8 + 3 pairs past a block without clamping.
+constexpr int kPairPad = 64;
 
 // Blocks for a grid-stride loop over n items. HIP limits one grid dimension to
 // fewer than 2^32 work-items, and the coefficient arrays of a 33^3 grid already

@@ -156,6 +156,17 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   const int cu = g.lo[0] + r;
   const int kk = lane >> 4, jj = lane & 15;
   const int col0 = wave * 16 * CT;
+  // Column jj of tile ct is cell col0 + CT jj + ct: a lane's CT staged values
+  // of one row are one 16-byte LDS read.
+  const int lcol = col0 + CT * jj;
+  // the lane's byte offsets inside a step of 4 pair records (rows 16 rt + jj
+  // of the coefficient operand, clamped to the code slot D, and the code)
+  uint32_t loffb[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
+  const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * 8);
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  static_assert(CT == 2, "one 16-byte LDS read per operand and step");
 
   dgv_d4 acc[RT][CT];
 #pragma unroll
@@ -163,6 +174,10 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = dgv_d4{0.0, 0.0, 0.0, 0.0};
 
+  const char* la_lane = reinterpret_cast<const char*>(lfa + lcol);
+  const char* lb_lane = reinterpret_cast<const char*>(lfb + lcol);
+  const char* ga_lane = reinterpret_cast<const char*>(lga + (MIX ? lcol : 0));
+  const char* gb_lane = reinterpret_cast<const char*>(lgb + (MIX ? lcol : 0));
   int prev_a = -1;   // offset a whose f is staged in lfa (a group's blocks come ordered by a)
   for (int bi = it.bbeg; bi < it.bend; ++bi) {
     const BlockHdr* B = blocks + gblocks[bi];
@@ -184,66 +199,77 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
     __syncthreads();
     const int n = B->npairs;
     if (n == 0) continue;
-    // Ring of kRing steps of coefficient loads in flight: unconditional global
-    // loads (indices clamped into the block's records; rows >= D read the code
-    // slot and only feed dropped output rows), steps past the end carry an
-    // invalid code and contribute 0.  The LDS reads of the next step's B
-    // operand are issued before the current step's products.
-    const AS1 double* P = (const AS1 double*)(pairs + B->pbeg * (D + 1));
+    // Ring of kRing steps of coefficient loads in flight.  The pair array
+    // carries zero records past its end, so the ring reads straight ahead from
+    // a uniform step base; only the block's last, partial step masks the pairs
+    // past the block's end.  The LDS reads of the next step's B operand are
+    // issued before the current step's products.
+    const AS1 char* P = (const AS1 char*)(pairs + B->pbeg * (D + 1));
+    constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
     double ab[kRing][RT];
     int cbits[kRing];
-    bool cok[kRing];
-#define PB_LOAD(slot, s)                                                                  \
+#define PB_LOAD(slot, step)                                                               \
   do {                                                                                    \
-    const int pv_ = (s) + kk;                                                             \
-    const AS1 double* R_ = P + (int64_t)min(pv_, n - 1) * (D + 1);                        \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = R_[min(16 * rt + jj, D)]; \
-    cbits[slot] = (int)__double_as_longlong(R_[D]);                                       \
-    cok[slot] = pv_ < n;                                                                  \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
+      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb[rt]);                               \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                         \
   } while (0)
-    double xa[CT], xb[CT], ya[MIX ? CT : 1], yb[MIX ? CT : 1], bcur[CT];
-    bool bok;
+    d2 xa, xb, ya, yb;
+    double bcur[CT];
 #define PB_ISSUE(slot)                                                                    \
   do {                                                                                    \
-    bok = cok[slot];                                                                      \
-    const int cz_ = bok ? cbits[slot] : 0;                                                \
-    const int p_ = cz_ >> 8, q_ = cz_ & 255;                                              \
-    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                   \
-      const int cell_ = col0 + 16 * ct + jj;                                              \
-      xa[ct] = lfa[p_ * 64 + cell_];                                                      \
-      xb[ct] = lfb[q_ * 64 + cell_];                                                      \
-      if constexpr (MIX) {                                                                \
-        ya[ct] = lga[p_ * 64 + cell_];                                                    \
-        yb[ct] = lgb[q_ * 64 + cell_];                                                    \
-      }                                                                                   \
+    const int c_ = cbits[slot];                                                           \
+    const int pb_ = (c_ >> 8) * 512, qb_ = (c_ & 255) * 512;                              \
+    xa = *reinterpret_cast<const d2*>(la_lane + pb_);                                     \
+    xb = *reinterpret_cast<const d2*>(lb_lane + qb_);                                     \
+    if constexpr (MIX) {                                                                  \
+      ya = *reinterpret_cast<const d2*>(ga_lane + pb_);                                   \
+      yb = *reinterpret_cast<const d2*>(gb_lane + qb_);                                   \
     }                                                                                     \
   } while (0)
 #define PB_FINISH()                                                                       \
   do {                                                                                    \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                   \
-      double prod_;                                                                       \
-      if constexpr (MIX) prod_ = __dadd_rn(__dmul_rn(ya[ct], xb[ct]), __dmul_rn(xa[ct], yb[ct])); \
-      else prod_ = xa[ct] * xb[ct];                                                       \
-      bcur[ct] = bok ? prod_ : 0.0;                                                       \
+      if constexpr (MIX) bcur[ct] = __dadd_rn(__dmul_rn(ya[ct], xb[ct]), __dmul_rn(xa[ct], yb[ct])); \
+      else bcur[ct] = xa[ct] * xb[ct];                                                    \
     }                                                                                     \
   } while (0)
+#define PB_MFMA(u)                                                                        \
+  do {                                                                                    \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                     \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
+      acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0); \
+  } while (0)
+    const int nfull = n >> 2, nst = (n + 3) >> 2;
 #pragma unroll
-    for (int u = 0; u < kRing; ++u) PB_LOAD(u, 4 * u);
+    for (int u = 0; u < kRing; ++u) PB_LOAD(u, u);
     PB_ISSUE(0);
     PB_FINISH();
-    for (int s0 = 0; s0 < n; s0 += 4 * kRing) {
+    int st = 0;
+    for (; st + kRing <= nfull; st += kRing) {
 #pragma unroll
       for (int u = 0; u < kRing; ++u) {
         PB_ISSUE((u + 1) % kRing);
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-            acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0);
-        PB_LOAD(u, s0 + 4 * u + 4 * kRing);
+        PB_MFMA(u);
+        PB_LOAD(u, st + u + kRing);
         PB_FINISH();
       }
     }
+#pragma unroll
+    for (int u = 0; u < kRing; ++u) {   // the last nst - st <= kRing steps, already in the ring
+      if (st + u < nst) {
+        if (st + u == nfull) {   // partial step: records past the block's end contribute 0
+          const bool ok = 4 * nfull + kk < n;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) ab[u][rt] = ok ? ab[u][rt] : 0.0;
+        }
+        PB_ISSUE((u + 1) % kRing);
+        PB_MFMA(u);
+        PB_FINISH();
+      }
+    }
+#undef PB_MFMA
 #undef PB_FINISH
 #undef PB_ISSUE
 #undef PB_LOAD
@@ -252,7 +278,7 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
-      const int cell = col0 + 16 * ct + jj;
+      const int cell = col0 + CT * jj + ct;
       if (it.chunk * 64 + cell >= g.ncells) continue;
       double* out = partial + (it.prow + cell) * D;
 #pragma unroll

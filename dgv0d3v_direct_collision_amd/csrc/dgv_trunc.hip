@@ -120,8 +120,12 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   }
   double* d_new = nullptr;
   int32_t* d_pos = nullptr;
-  if (hipMalloc(&d_new, sizeof(double) * std::max<int64_t>(next * (D + 1), 1)) != hipSuccess)
+  if (hipMalloc(&d_new, sizeof(double) * (next + kPairPad) * (D + 1)) != hipSuccess)
     return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(truncated pairs) failed");
+  if (hipMemsetAsync(d_new + next * (D + 1), 0, sizeof(double) * kPairPad * (D + 1), ctx->stream) != hipSuccess) {
+    hipFree(d_new);
+    return set_error(ctx, DGV_ERR_HIP, "hipMemsetAsync(truncated pairs padding) failed");
+  }
   if (hipMalloc(&d_pos, sizeof(int32_t) * std::max<int64_t>(np, 1)) != hipSuccess) {
     hipFree(d_new);
     return set_error(ctx, DGV_ERR_NOMEM, "hipMalloc(truncation map) failed");
