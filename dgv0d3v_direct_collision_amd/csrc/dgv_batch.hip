@@ -206,6 +206,11 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 2) + min(16 * rt + jj, D)) * (int)sizeof(float));
   const uint32_t coffb = (uint32_t)((kk * (D + 2) + D) * (int)sizeof(float));
+  // FP64: the same inside a step of the context's pair records (D + 1 doubles each)
+  uint32_t loffb64[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) loffb64[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
+  const uint32_t coffb64 = (uint32_t)((kk * (D + 1) + D) * 8);
   const V4 vzero = V4{0, 0, 0, 0};
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -367,48 +372,50 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #undef FINISH32
 #undef LOAD32
       } else {
-        // FP64: the coefficients are the context's own array (no padding), so the
-        // ring's indices are clamped into the block and steps past its end carry
-        // a zero B operand.
-        const AS1 T* P = (const AS1 T*)(G.pairs64 + B->pbeg * (D + 1));
-        bool cok[kRing];
-        bool bok;
-#define LOAD64(slot, s)                                                                       \
+        // FP64: the context's pair array (zero records past its end), read
+        // straight ahead like the FP32 copy; the sums of a block go into the
+        // running FP64 total after the block.
+        const AS1 char* P = (const AS1 char*)(G.pairs64 + B->pbeg * (D + 1));
+        constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
+#define LOAD64(slot, step)                                                                    \
   do {                                                                                        \
-    const int pv_ = (s) + kk;                                                                 \
-    const AS1 T* R_ = P + (int64_t)min(pv_, n - 1) * (D + 1);                                 \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = R_[min(16 * rt + jj, D)]; \
-    cbits[slot] = (int)__double_as_longlong(R_[D]);                                           \
-    cok[slot] = pv_ < n;                                                                      \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb64[rt]);                                 \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb64);                                           \
   } while (0)
 #define ISSUE64(slot)                                                                         \
   do {                                                                                        \
-    bok = cok[slot];                                                                          \
-    const int c_ = bok ? cbits[slot] : 0;                                                     \
+    const int c_ = cbits[slot];                                                               \
     B_ISSUE_BYTES((c_ >> 8) * 64 * (int)sizeof(T), (c_ & 255) * 64 * (int)sizeof(T));        \
   } while (0)
 #define FINISH64()                                                                            \
   do {                                                                                        \
-    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = bok ? ra[ct] * rb[ct] : (T)0; \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = ra[ct] * rb[ct];             \
   } while (0)
-        const int nst = (n + 3) >> 2;
+        const int nfull = n >> 2, nst = (n + 3) >> 2;
 #pragma unroll
-        for (int u = 0; u < kRing; ++u) LOAD64(u, 4 * u);
+        for (int u = 0; u < kRing; ++u) LOAD64(u, u);
         ISSUE64(0);
         FINISH64();
         int st = 0;
-        for (; st + kRing <= nst; st += kRing) {
+        for (; st + kRing <= nfull; st += kRing) {
 #pragma unroll
           for (int u = 0; u < kRing; ++u) {
             ISSUE64((u + 1) % kRing);
             STEP_MFMA(u, false);
-            LOAD64(u, 4 * (st + u + kRing));
+            LOAD64(u, st + u + kRing);
             FINISH64();
           }
         }
 #pragma unroll
         for (int u = 0; u < kRing; ++u) {
           if (st + u < nst) {
+            if (st + u == nfull) {   // partial step: records past the block's end contribute 0
+              const bool ok = 4 * nfull + kk < n;
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) ab[u][rt] = ok ? ab[u][rt] : (T)0;
+            }
             ISSUE64((u + 1) % kRing);
             STEP_MFMA(u, false);
             FINISH64();

@@ -181,3 +181,32 @@ def test_batch_device_rows_match_per_instance_copies(grids):
     g.device_free(d_rows)
     g.device_free(d_q)
     b.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("FP64", 1e-12), ("FP32_COMP", 1e-5)])
+def test_two_nodes_per_dimension_batch_matches_single(precision, tol):
+    """D = 8 (s = 2) grids, synthetic A-arrays with uneven blocks: the batched
+    kernels' D = 8 instantiation against the single-VDF evaluator."""
+    import dgv0d3v_direct_collision_amd as dgv
+    import oracle
+    from tests.helpers import synthetic_aarray
+    print(f"\n[batch] s=2 grids M=4 and M=5, {precision}", flush=True)
+    gs = []
+    for M, box, seed in ((4, (-2.0, 2.0), 5), (5, (-2.5, 2.5), 6)):
+        G = oracle.Geometry(M, 2, box)
+        g = dgv.DGV(M, 2, box)
+        g.set_aarray(*synthetic_aarray(G, seed=seed))
+        gs.append(g)
+    rng = np.random.default_rng(12)
+    which = np.array([0, 1] * 40 + [1] * 7, np.int32)
+    b = dgv.DGVBatch(gs, which, getattr(dgv, "DGV_" + precision))
+    fs = [vdf(gs[g], rng, 1 + i % 3) for i, g in enumerate(which)]
+    for i, f in enumerate(fs):
+        b.set_f(i, f)
+    b.eval_q()
+    worst = max(rel_l2(b.get_q(i), gs[g].EvalCollisionPeriodicAPlus_DGV(fs[i])) for i, g in enumerate(which))
+    print(f"[batch] s=2 {precision}: worst rel L2 {worst:.2e}", flush=True)
+    assert worst <= tol
+    b.close()
+    for g in gs:
+        g.close()
