@@ -267,10 +267,24 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #pragma unroll
           for (int q = 0; q < D; ++q) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
         } else {
+          // waves 2c and 2c+1 split the rows of chunk c; all loads first
           const double* F = f64 + G.foff;
-          if (new_x)
-            for (int p = sub; p < D; p += WPC) la[p * 64 + lane] = F[((int64_t)xc * D + p) * G.Bpad + vb];
-          for (int q = sub; q < D; q += WPC) lb[q * 64 + lane] = F[((int64_t)yc * D + q) * G.Bpad + vb];
+          constexpr int NS = (D + WPC - 1) / WPC;
+          double sa[NS], sb[NS];
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            const int p = min(sub + i * WPC, D - 1);
+            if (new_x) sa[i] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+            sb[i] = F[((int64_t)yc * D + p) * G.Bpad + vb];
+          }
+#pragma unroll
+          for (int i = 0; i < NS; ++i) {
+            const int p = sub + i * WPC;
+            if (p < D) {
+              if (new_x) la[p * 64 + lane] = sa[i];
+              lb[p * 64 + lane] = sb[i];
+            }
+          }
         }
       }
       __syncthreads();

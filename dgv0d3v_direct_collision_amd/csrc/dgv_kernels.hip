@@ -129,6 +129,10 @@ typedef double dgv_d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPbWaves = 2;
 
+typedef double dgv_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double lane_el(const dgv_d2& v, int i) { return v[i]; }
+__device__ __forceinline__ double lane_el(double v, int) { return v; }
+
 template <int D, bool MIX>
 __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __restrict__ items,
                                                            const Group* __restrict__ groups,
@@ -166,7 +170,8 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
   const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * 8);
   typedef double d2 __attribute__((ext_vector_type(2)));
-  static_assert(CT == 2, "one 16-byte LDS read per operand and step");
+  using OpT = typename std::conditional<CT == 2, d2, double>::type;   // a lane's CT values of a row
+  static_assert(CT == 1 || CT == 2, "one LDS read per operand and step");
 
   dgv_d4 acc[RT][CT];
 #pragma unroll
@@ -186,16 +191,38 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
     const int akey = ((B->a[0] + 64) << 16) | ((B->a[1] + 64) << 8) | (B->a[2] + 64);
     const bool new_a = akey != prev_a;
     prev_a = akey;
+    // thread (wave, lane) stages the nodes p = wave, wave + kPbWaves, ... of its
+    // lane's cell: all loads are issued into registers first (one memory
+    // latency per block), then written to LDS after the barrier
+    constexpr int NS = (D + kPbWaves - 1) / kPbWaves;
+    double sa[NS], sb[NS];
+#define PB_STAGE_LOAD(src)                                                                \
+  do {                                                                                    \
+    _Pragma("unroll") for (int i = 0; i < NS; ++i) {                                      \
+      const int p_ = min(wave + i * kPbWaves, D - 1);                                     \
+      if (new_a) sa[i] = (src)[(int64_t)xc * D + p_];                                     \
+      sb[i] = (src)[(int64_t)yc * D + p_];                                                \
+    }                                                                                     \
+  } while (0)
+#define PB_STAGE_STORE(da, db)                                                            \
+  do {                                                                                    \
+    _Pragma("unroll") for (int i = 0; i < NS; ++i) {                                      \
+      const int p_ = wave + i * kPbWaves;                                                 \
+      if (p_ < D) {                                                                       \
+        if (new_a) (da)[p_ * 64 + lane] = sa[i];                                          \
+        (db)[p_ * 64 + lane] = sb[i];                                                     \
+      }                                                                                   \
+    }                                                                                     \
+  } while (0)
+    PB_STAGE_LOAD(f);
     __syncthreads();  // previous block's LDS reads are done
-    // thread (wave, lane) stages the nodes p = wave, wave + 2, ... of its lane's cell
-    if (new_a)
-      for (int p = wave; p < D; p += kPbWaves) lfa[p * 64 + lane] = f[(int64_t)xc * D + p];
-    for (int q = wave; q < D; q += kPbWaves) lfb[q * 64 + lane] = f[(int64_t)yc * D + q];
-    if constexpr (MIX) {
-      if (new_a)
-        for (int p = wave; p < D; p += kPbWaves) lga[p * 64 + lane] = gm[(int64_t)xc * D + p];
-      for (int q = wave; q < D; q += kPbWaves) lgb[q * 64 + lane] = gm[(int64_t)yc * D + q];
+    PB_STAGE_STORE(lfa, lfb);
+    if constexpr (MIX) {   // the second VDF through the same registers
+      PB_STAGE_LOAD(gm);
+      PB_STAGE_STORE(lga, lgb);
     }
+#undef PB_STAGE_STORE
+#undef PB_STAGE_LOAD
     __syncthreads();
     const int n = B->npairs;
     if (n == 0) continue;
@@ -215,24 +242,25 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
       ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb[rt]);                               \
     cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                         \
   } while (0)
-    d2 xa, xb, ya, yb;
+    OpT xa, xb, ya, yb;
     double bcur[CT];
 #define PB_ISSUE(slot)                                                                    \
   do {                                                                                    \
     const int c_ = cbits[slot];                                                           \
     const int pb_ = (c_ >> 8) * 512, qb_ = (c_ & 255) * 512;                              \
-    xa = *reinterpret_cast<const d2*>(la_lane + pb_);                                     \
-    xb = *reinterpret_cast<const d2*>(lb_lane + qb_);                                     \
+    xa = *reinterpret_cast<const OpT*>(la_lane + pb_);                                    \
+    xb = *reinterpret_cast<const OpT*>(lb_lane + qb_);                                    \
     if constexpr (MIX) {                                                                  \
-      ya = *reinterpret_cast<const d2*>(ga_lane + pb_);                                   \
-      yb = *reinterpret_cast<const d2*>(gb_lane + qb_);                                   \
+      ya = *reinterpret_cast<const OpT*>(ga_lane + pb_);                                  \
+      yb = *reinterpret_cast<const OpT*>(gb_lane + qb_);                                  \
     }                                                                                     \
   } while (0)
 #define PB_FINISH()                                                                       \
   do {                                                                                    \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                   \
-      if constexpr (MIX) bcur[ct] = __dadd_rn(__dmul_rn(ya[ct], xb[ct]), __dmul_rn(xa[ct], yb[ct])); \
-      else bcur[ct] = xa[ct] * xb[ct];                                                    \
+      if constexpr (MIX)                                                                  \
+        bcur[ct] = __dadd_rn(__dmul_rn(lane_el(ya, ct), lane_el(xb, ct)), __dmul_rn(lane_el(xa, ct), lane_el(yb, ct))); \
+      else bcur[ct] = lane_el(xa, ct) * lane_el(xb, ct);                                  \
     }                                                                                     \
   } while (0)
 #define PB_MFMA(u)                                                                        \
