@@ -163,6 +163,7 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   // Column jj of tile ct is cell col0 + CT jj + ct: a lane's CT staged values
   // of one row are one 16-byte LDS read.
   const int lcol = col0 + CT * jj;
+  const bool wave_cells = it.chunk * 64 + col0 < g.ncells;   // the wave owns at least one cell
   // the lane's byte offsets inside a step of 4 pair records (rows 16 rt + jj
   // of the coefficient operand, clamped to the code slot D, and the code)
   uint32_t loffb[RT];
@@ -225,7 +226,9 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
 #undef PB_STAGE_LOAD
     __syncthreads();
     const int n = B->npairs;
-    if (n == 0) continue;
+    // a wave whose cells all lie past the group's box (a chunk's partial end)
+    // only stages: its products would all be dropped
+    if (n == 0 || !wave_cells) continue;
     // Ring of kRing steps of coefficient loads in flight.  The pair array
     // carries zero records past its end, so the ring reads straight ahead from
     // a uniform step base; only the block's last, partial step masks the pairs
