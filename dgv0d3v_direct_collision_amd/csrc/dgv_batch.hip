@@ -696,6 +696,26 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
   // keep the partial sums within 4 GiB: coarser segments otherwise
   if (nrows * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30))
     return build_batch_plan(b, P, tile * 2, cells);
+  // Execution order: by the position of the item's first block in the grid's
+  // block order, then by cell.  Items that run at the same time then stream the
+  // same coefficient records for different cells (shared through L2); each
+  // item keeps its partial-sum rows, so the reduction order is unchanged.
+  {
+    std::vector<int64_t> key1(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+      const BatchItem& it = items[i];
+      key1[i] = ((int64_t)it.grid << 40) | ((int64_t)b->h_blist[it.grid][it.b0] << 8);
+    }
+    std::vector<size_t> order(items.size());
+    std::iota(order.begin(), order.end(), (size_t)0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+      if (key1[x] != key1[y]) return key1[x] < key1[y];
+      return items[x].cell < items[y].cell;
+    });
+    std::vector<BatchItem> sorted(items.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    items.swap(sorted);
+  }
   P.items.swap(items);
   P.nrows = nrows;
   P.nred = (int64_t)rptr.size() - 1;
