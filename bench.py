@@ -313,13 +313,16 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
                      "traffic": measured_traffic("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
+                     "l2": measured_l2("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
                      "kernel": "k_batch_mfma<27,FP32>", "kernel_ms": r32["kernel_ms"],
                      "flops_per_launch": flops,
                      "hbm_bytes_per_launch_algorithmic": 12.0 * R + 16.0 * N * B,
                      "fp64_plain": {"kernel": knames[best], "kernel_ms": ms_single_main,
                                     "tflops": 2.0 * V / (ms_single_main * 1e-3) / 1e12,
                                     "frac_fp64_peak": 2.0 * V / (ms_single_main * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                                    "hbm_gbs_algorithmic": (16.0 * R + 16.0 * N) / (ms_single_main * 1e-3) / 1e9}},
+                                    "hbm_gbs_algorithmic": (16.0 * R + 16.0 * N) / (ms_single_main * 1e-3) / 1e9,
+                                    "traffic": measured_traffic("k_pb_mfma<27,false>", f"single_M{M}"),
+                                    "l2": measured_l2("k_pb_mfma<27,false>", f"single_M{M}")}},
         "cpu_baseline": cpu,
         "conservation": {"sum_w_psi_Q": [float(x) for x in mom], "sum_abs_w_psi_Q": absmom,
                          "after_projection_rel_max": proj_rel},
@@ -566,6 +569,24 @@ def measured_traffic(kernel: str, workload: str):
                 return float(kv["fetch_kib"]) * 1024 / float(kv["copy_fetch_factor"]) + \
                     float(kv["write_kib"]) * 1024 / float(kv["copy_write_factor"])
             except (KeyError, ValueError):
+                return None
+    return None
+
+
+def measured_l2(kernel: str, workload: str):
+    """L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS) of `kernel` and its read
+    requests past L2 per launch, from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        kv = dict(t.split("=", 1) for t in line.split() if "=" in t)
+        if kv.get("l2kernel") == kernel and kv.get("workload") == workload:
+            try:
+                h, m = float(kv["tcc_hit"]), float(kv["tcc_miss"])
+                return {"l2_hit_rate": h / (h + m), "tcc_hit": h, "tcc_miss": m,
+                        "ea_read_requests": float(kv["tcc_ea0_rdreq"])}
+            except (KeyError, ValueError, ZeroDivisionError):
                 return None
     return None
 
