@@ -256,6 +256,44 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
       const bool new_x = xc != prev_xc;
       prev_xc = xc;
+      const int n = B->npairs;
+      // Coefficient operand of step s (4 pairs): lane i holds T[pair 4s + i/16][16 rt + i%16]
+      // (rows >= D read the pair's code slot and only feed output rows that are
+      // dropped).  Both record arrays carry zero pairs past their end, so the
+      // ring of kRing steps in flight reads straight ahead -- a uniform step base
+      // plus the lane's fixed byte offset, no clamping; only the block's last,
+      // partial step masks the pairs past its end.  The ring is filled before
+      // the f staging, so the block's two memory latencies overlap.
+      constexpr int kRing = 8;
+      T ab[kRing][RT];
+      int cbits[kRing];
+      int qbits[FP32 ? kRing : 1];
+      const AS1 char* P;
+      if constexpr (FP32) P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 2));
+      else P = (const AS1 char*)(G.pairs64 + B->pbeg * (D + 1));
+      constexpr int64_t kStepBytes = FP32 ? 4 * (D + 2) * (int64_t)sizeof(float) : 4 * (D + 1) * 8;
+#define LOAD32(slot, step)                                                                    \
+  do {                                                                                        \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+      ab[slot][rt] = *(const AS1 float*)(Ps_ + loffb[rt]);                                    \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                             \
+    qbits[slot] = *(const AS1 int*)(Ps_ + coffb + 4);                                         \
+  } while (0)
+#define LOAD64(slot, step)                                                                    \
+  do {                                                                                        \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
+      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb64[rt]);                                 \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb64);                                           \
+  } while (0)
+      if (active && n > 0) {
+#pragma unroll
+        for (int u = 0; u < kRing; ++u) {
+          if constexpr (FP32) LOAD32(u, u);
+          else LOAD64(u, u);
+        }
+      }
       __syncthreads();
       if (active) {
         if constexpr (FP32) {
@@ -288,11 +326,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         }
       }
       __syncthreads();
-      const int n = B->npairs;
       if (!active || n == 0) continue;
-      constexpr int kRing = 8;
-      T ab[kRing][RT];
-      int cbits[kRing];
       VB ra, rb;
       T bcur[CT];
       const char* la_lane = reinterpret_cast<const char*>(la + lcol);
@@ -318,30 +352,11 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
     }                                                                                         \
   } while (0)
       if constexpr (FP32) {
-        // Coefficient operand of step s (4 pairs): lane i holds T[pair 4s + i/16][16 rt + i%16]
-        // (rows >= D read the pair's code slot and only feed output rows that are
-        // dropped).  The FP32 copy of the coefficients carries zero pairs past its
-        // end, so the ring of kRing steps in flight reads straight ahead -- a
-        // uniform step base plus the lane's fixed byte offset, no clamping; only
-        // the block's last, partial step masks the pairs past its end.
-        const AS1 char* P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 2));
-        constexpr int64_t kStepBytes = 4 * (D + 2) * (int64_t)sizeof(float);
-        int qbits[kRing];
-#define LOAD32(slot, step)                                                                    \
-  do {                                                                                        \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
-      ab[slot][rt] = *(const AS1 float*)(Ps_ + loffb[rt]);                                    \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                             \
-    qbits[slot] = *(const AS1 int*)(Ps_ + coffb + 4);                                         \
-  } while (0)
 #define FINISH32()                                                                            \
   do {                                                                                        \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = ra[ct] * rb[ct];             \
   } while (0)
         const int nfull = n >> 2, nst = (n + 3) >> 2;
-#pragma unroll
-        for (int u = 0; u < kRing; ++u) LOAD32(u, u);
         B_ISSUE_BYTES(cbits[0], qbits[0]);
         FINISH32();
 #define ROUND32(st_)                                                                           \
@@ -384,20 +399,9 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
           FOLD();
         }
 #undef FINISH32
-#undef LOAD32
       } else {
-        // FP64: the context's pair array (zero records past its end), read
-        // straight ahead like the FP32 copy; the sums of a block go into the
-        // running FP64 total after the block.
-        const AS1 char* P = (const AS1 char*)(G.pairs64 + B->pbeg * (D + 1));
-        constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
-#define LOAD64(slot, step)                                                                    \
-  do {                                                                                        \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
-      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb64[rt]);                                 \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb64);                                           \
-  } while (0)
+        // FP64: the context's pair array, read straight ahead like the FP32
+        // copy; the sums of a block go into the running FP64 total after it.
 #define ISSUE64(slot)                                                                         \
   do {                                                                                        \
     const int c_ = cbits[slot];                                                               \
@@ -408,8 +412,6 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) bcur[ct] = ra[ct] * rb[ct];             \
   } while (0)
         const int nfull = n >> 2, nst = (n + 3) >> 2;
-#pragma unroll
-        for (int u = 0; u < kRing; ++u) LOAD64(u, u);
         ISSUE64(0);
         FINISH64();
         int st = 0;
@@ -437,10 +439,11 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         }
 #undef FINISH64
 #undef ISSUE64
-#undef LOAD64
       }
 #undef STEP_MFMA
 #undef B_ISSUE_BYTES
+#undef LOAD64
+#undef LOAD32
       if constexpr (!FP32) {  // two-level FP64 sum
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)

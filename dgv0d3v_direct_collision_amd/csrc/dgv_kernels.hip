@@ -192,6 +192,30 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
     const int akey = ((B->a[0] + 64) << 16) | ((B->a[1] + 64) << 8) | (B->a[2] + 64);
     const bool new_a = akey != prev_a;
     prev_a = akey;
+    const int n = B->npairs;
+    // a wave whose cells all lie past the group's box (a chunk's partial end)
+    // only stages: its products would all be dropped
+    const bool compute = n > 0 && wave_cells;
+    // Ring of kRing steps of coefficient loads in flight.  The pair array
+    // carries zero records past its end, so the ring reads straight ahead from
+    // a uniform step base; only the block's last, partial step masks the pairs
+    // past the block's end.  The ring is filled before the f staging, so the
+    // two memory latencies of a block overlap.
+    const AS1 char* P = (const AS1 char*)(pairs + B->pbeg * (D + 1));
+    constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
+    double ab[kRing][RT];
+    int cbits[kRing];
+#define PB_LOAD(slot, step)                                                               \
+  do {                                                                                    \
+    const AS1 char* Ps_ = P + kStepBytes * (step);                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
+      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb[rt]);                               \
+    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                         \
+  } while (0)
+    if (compute) {
+#pragma unroll
+      for (int u = 0; u < kRing; ++u) PB_LOAD(u, u);
+    }
     // thread (wave, lane) stages the nodes p = wave, wave + kPbWaves, ... of its
     // lane's cell: all loads are issued into registers first (one memory
     // latency per block), then written to LDS after the barrier
@@ -225,26 +249,9 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
 #undef PB_STAGE_STORE
 #undef PB_STAGE_LOAD
     __syncthreads();
-    const int n = B->npairs;
-    // a wave whose cells all lie past the group's box (a chunk's partial end)
-    // only stages: its products would all be dropped
-    if (n == 0 || !wave_cells) continue;
-    // Ring of kRing steps of coefficient loads in flight.  The pair array
-    // carries zero records past its end, so the ring reads straight ahead from
-    // a uniform step base; only the block's last, partial step masks the pairs
-    // past the block's end.  The LDS reads of the next step's B operand are
-    // issued before the current step's products.
-    const AS1 char* P = (const AS1 char*)(pairs + B->pbeg * (D + 1));
-    constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
-    double ab[kRing][RT];
-    int cbits[kRing];
-#define PB_LOAD(slot, step)                                                               \
-  do {                                                                                    \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                        \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
-      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb[rt]);                               \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                         \
-  } while (0)
+    if (!compute) continue;
+    // The LDS reads of the next step's B operand are issued before the current
+    // step's products.
     OpT xa, xb, ya, yb;
     double bcur[CT];
 #define PB_ISSUE(slot)                                                                    \
@@ -273,8 +280,6 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
       acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u][rt], bcur[ct], acc[rt][ct], 0, 0, 0); \
   } while (0)
     const int nfull = n >> 2, nst = (n + 3) >> 2;
-#pragma unroll
-    for (int u = 0; u < kRing; ++u) PB_LOAD(u, u);
     PB_ISSUE(0);
     PB_FINISH();
     int st = 0;
