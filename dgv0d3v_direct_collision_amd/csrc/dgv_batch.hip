@@ -177,6 +177,18 @@ constexpr int kPadPairs = 64;   // zero pairs after the FP32 coefficients (>= 4 
 template <bool FP32>
 constexpr int chunks_per_item() { return FP32 ? 4 : 2; }
 
+// FP32 row map of the matrix-core tiles: tile 0 holds basis functions 0..15 in
+// rows 0..15; tile 1 holds function 16 + 3 (i / 4) + i % 4 in row i for
+// i % 4 < 3.  Every unused row (i % 4 == 3 in tile 1, or a function past D,
+// both mapped to D) then falls in accumulator element 3 of tile 1 (the FP32
+// output row of element e is 4 (lane / 16) + e), which the fold skips.
+template <int D>
+__device__ __forceinline__ int f32_row_fn(int rt, int i) {
+  static_assert(D <= 28, "tile 0 holds 16 functions, tile 1 holds 12");
+  const int l = rt == 0 ? i : ((i & 3) < 3 ? 16 + 3 * (i >> 2) + (i & 3) : D);
+  return min(l, D);
+}
+
 template <int D, bool FP32>
 __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
                                                                    const BatchGridDev* __restrict__ grids,
@@ -200,11 +212,12 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   const int sub = wave % WPC;                    // its part of the chunk's staging / columns
   const int col0 = sub * 16 * CT;
   const int lcol = col0 + CT * jj;               // the lane's first instance in a staged row
-  // FP32: the lane's byte offsets inside a step of 4 pairs (coefficient rows
-  // 16 rt + jj, clamped to the code slot D, and the pair's code)
+  // FP32: the lane's byte offsets inside a step of 4 pairs (the coefficient of
+  // its tile rows' basis functions, unused rows on the code slot D, and the
+  // pair's code)
   uint32_t loffb[RT];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 2) + min(16 * rt + jj, D)) * (int)sizeof(float));
+  for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 2) + f32_row_fn<D>(rt, jj)) * (int)sizeof(float));
   const uint32_t coffb = (uint32_t)((kk * (D + 2) + D) * (int)sizeof(float));
   // FP64: the same inside a step of the context's pair records (D + 1 doubles each)
   uint32_t loffb64[RT];
@@ -238,12 +251,13 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
     T* lb = lfb[ch];
     int prev_xc = -1;   // the cell whose f is in la (blocks come ordered by a)
     // fold a round's matrix-core sums into the FP32 (hi, lo) pair (the next
-    // round's first products start a fresh sum, so acc needs no clearing)
+    // round's first products start a fresh sum, so acc needs no clearing);
+    // element 3 of tile 1 holds only unused rows (f32_row_fn) and is skipped
 #define FOLD()                                                                                \
   do {                                                                                        \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                         \
-    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                           \
+    _Pragma("unroll") for (int e = 0; e < (rt == 0 ? 4 : 3); ++e) {                           \
       float s2_, er_;                                                                         \
       two_sum(hi[rt][ct][e], acc[rt][ct][e], s2_, er_);                                       \
       hi[rt][ct][e] = s2_;                                                                    \
@@ -465,7 +479,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           // output layouts: FP64 row 4e + lane/16, FP32 row 4 (lane/16) + e
-          const int l = FP32 ? 16 * rt + 4 * kk + e : 16 * rt + 4 * e + kk;
+          const int l = FP32 ? f32_row_fn<D>(rt, 4 * kk + e) : 16 * rt + 4 * e + kk;
           if (l < D) {
             if constexpr (FP32) out[l] = (double)hi[rt][ct][e] + (double)lo[rt][ct][e];
             else out[l] = hi[rt][ct][e];
