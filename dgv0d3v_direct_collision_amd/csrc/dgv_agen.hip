@@ -169,7 +169,7 @@ __global__ __launch_bounds__(128) void k_integrate(AgenConsts C, int64_t npairs,
     }
     out[l] = a;
   }
-  out[D] = __longlong_as_double((long long)((p << 8) | q));  // pair code
+  out[D] = __longlong_as_double((long long)pair_code_bits(p, q));
   keep[k] = any ? 1 : 0;
 }
 

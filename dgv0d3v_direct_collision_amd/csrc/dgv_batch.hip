@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   uint32_t loffb64[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) loffb64[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
-  const uint32_t coffb64 = (uint32_t)((kk * (D + 1) + D) * 8);
+  const uint32_t coffb64 = (uint32_t)((kk * (D + 1) + D) * 8 + 4);   // the LDS offsets (pair_code_bits)
   const V4 vzero = V4{0, 0, 0, 0};
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -418,8 +418,8 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         // copy; the sums of a block go into the running FP64 total after it.
 #define ISSUE64(slot)                                                                         \
   do {                                                                                        \
-    const int c_ = cbits[slot];                                                               \
-    B_ISSUE_BYTES((c_ >> 8) * 64 * (int)sizeof(T), (c_ & 255) * 64 * (int)sizeof(T));        \
+    const uint32_t c_ = (uint32_t)cbits[slot];                                                \
+    B_ISSUE_BYTES(c_ & 0xffffu, c_ >> 16);                                                    \
   } while (0)
 #define FINISH64()                                                                            \
   do {                                                                                        \

@@ -361,7 +361,7 @@ int build_pairblock_from_records(dgv_ctx* ctx, const int32_t* capphi, int64_t nc
       pairs[base + ent[j].l] += ent[j].A;
       ++j;
     }
-    int64_t qbits = (int64_t)((((ent[i].key >> 6) & 63) << 8) | (ent[i].key & 63));  // pair code p<<8|q
+    int64_t qbits = pair_code_bits((int)((ent[i].key >> 6) & 63), (int)(ent[i].key & 63));
     std::memcpy(&pairs[base + D], &qbits, sizeof(double));
     i = j;
   }

@@ -10,7 +10,9 @@
 // records are therefore regrouped by pair:
 //
 //   pair   = (T[0..D-1], code)     T[l] = A of basis l (0 when l has no record),
-//                                   code = p << 8 | q, p / q = local index of xi / xi1;
+//                                   code = p << 8 | q, p / q = local index of xi / xi1
+//                                   (low word of the slot; the high word holds the
+//                                   same pair as LDS byte offsets, pair_code_bits);
 //   block  = all pairs whose xi lies in cell (cc + a) and xi1 in cell (cc + b),
 //            sorted by p = local index of xi; seg[p] = first pair with index >= p;
 //   group  = all blocks sharing the same box of output cells c for which both
@@ -35,6 +37,16 @@ constexpr int kMaxS = 8;
 // Zero pairs after every pair-record array on the device: the evaluators' load
 // rings read up to 4 * 8 + 3 pairs past a block without clamping.
 constexpr int kPairPad = 64;
+
+// The 64-bit code slot of a pair record.  Low word: p << 8 | q.  High word: the
+// byte offsets of rows p and q in the FP64 evaluators' LDS staging (64 doubles
+// per row), p * 512 in bits 0..15 and q * 512 in bits 16..31, so that a kernel
+// forms both operand addresses with one add each.
+__host__ __device__ inline int64_t pair_code_bits(int p, int q) {
+  const uint64_t lo = (uint32_t)((p << 8) | q);
+  const uint64_t hi = (uint32_t)(p * 512) | ((uint32_t)(q * 512) << 16);
+  return (int64_t)((hi << 32) | lo);
+}
 
 // Blocks for a grid-stride loop over n items. HIP limits one grid dimension to
 // fewer than 2^32 work-items, and the coefficient arrays of a 33^3 grid already

@@ -165,11 +165,12 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   const int lcol = col0 + CT * jj;
   const bool wave_cells = it.chunk * 64 + col0 < g.ncells;   // the wave owns at least one cell
   // the lane's byte offsets inside a step of 4 pair records (rows 16 rt + jj
-  // of the coefficient operand, clamped to the code slot D, and the code)
+  // of the coefficient operand, clamped to the code slot D, and the high word
+  // of the code slot: the pair's two LDS row offsets)
   uint32_t loffb[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) loffb[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
-  const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * 8);
+  const uint32_t coffb = (uint32_t)((kk * (D + 1) + D) * 8 + 4);
   typedef double d2 __attribute__((ext_vector_type(2)));
   using OpT = typename std::conditional<CT == 2, d2, double>::type;   // a lane's CT values of a row
   static_assert(CT == 1 || CT == 2, "one LDS read per operand and step");
@@ -256,8 +257,8 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
     double bcur[CT];
 #define PB_ISSUE(slot)                                                                    \
   do {                                                                                    \
-    const int c_ = cbits[slot];                                                           \
-    const int pb_ = (c_ >> 8) * 512, qb_ = (c_ & 255) * 512;                              \
+    const uint32_t c_ = (uint32_t)cbits[slot];                                            \
+    const uint32_t pb_ = c_ & 0xffffu, qb_ = c_ >> 16;   /* pair_code_bits */             \
     xa = *reinterpret_cast<const OpT*>(la_lane + pb_);                                    \
     xb = *reinterpret_cast<const OpT*>(lb_lane + qb_);                                    \
     if constexpr (MIX) {                                                                  \
