@@ -228,7 +228,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int id = s_item;
+    const int id = __builtin_amdgcn_readfirstlane(s_item);   // the workgroup's item, uniform
     __syncthreads();
     if (id >= nitems) return;
     const BatchItem it = items[id];
@@ -282,24 +282,20 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       T ab[kRing][RT];
       int cbits[kRing];
       int qbits[FP32 ? kRing : 1];
-      const AS1 char* P;
-      if constexpr (FP32) P = (const AS1 char*)(G.pairs32 + B->pbeg * (D + 2));
-      else P = (const AS1 char*)(G.pairs64 + B->pbeg * (D + 1));
-      constexpr int64_t kStepBytes = FP32 ? 4 * (D + 2) * (int64_t)sizeof(float) : 4 * (D + 1) * 8;
+      const __amdgpu_buffer_rsrc_t P =
+          ring_rsrc(FP32 ? (const void*)(G.pairs32 + B->pbeg * (D + 2)) : (const void*)(G.pairs64 + B->pbeg * (D + 1)));
+      constexpr uint32_t kStepBytes = FP32 ? 4 * (D + 2) * sizeof(float) : 4 * (D + 1) * 8;
 #define LOAD32(slot, step)                                                                    \
   do {                                                                                        \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
-      ab[slot][rt] = *(const AS1 float*)(Ps_ + loffb[rt]);                                    \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                             \
-    qbits[slot] = *(const AS1 int*)(Ps_ + coffb + 4);                                         \
+    const uint32_t s_ = kStepBytes * (uint32_t)(step);                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = ring_f32(P, loffb[rt], s_); \
+    ring_i32x2(P, coffb, s_, cbits[slot], qbits[slot]);                                       \
   } while (0)
 #define LOAD64(slot, step)                                                                    \
   do {                                                                                        \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                            \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
-      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb64[rt]);                                 \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb64);                                           \
+    const uint32_t s_ = kStepBytes * (uint32_t)(step);                                        \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = ring_f64(P, loffb64[rt], s_); \
+    cbits[slot] = ring_i32(P, coffb64, s_);                                                   \
   } while (0)
       if (active && n > 0) {
 #pragma unroll

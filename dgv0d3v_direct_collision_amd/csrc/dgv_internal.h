@@ -48,6 +48,38 @@ __host__ __device__ inline int64_t pair_code_bits(int p, int q) {
   return (int64_t)((hi << 32) | lo);
 }
 
+// The evaluators' coefficient rings read a block's pair records through a
+// buffer resource based at the block: the step is a scalar offset and the
+// lane's part a fixed 32-bit offset, so a load costs no 64-bit address
+// arithmetic on the VALU.  The resource does no range check (maximum record
+// count); the kPairPad zero records after every pair array keep the rings'
+// reads, at most 4 * 8 + 3 pairs past a block, inside the allocation.  The
+// base must be the same in every lane of the wave (a block of the workgroup's
+// item); it is read from the first lane, so that the resource lives in scalar
+// registers even where the compiler cannot prove the base uniform.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ring_rsrc(const void* block_base) {
+  const uint64_t a = reinterpret_cast<uint64_t>(block_base);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+  void* base = reinterpret_cast<void*>(((uint64_t)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ double ring_f64(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t step_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, step_off, 0));
+}
+__device__ __forceinline__ float ring_f32(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t step_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, step_off, 0));
+}
+__device__ __forceinline__ int ring_i32(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t step_off) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, lane_off, step_off, 0);
+}
+__device__ __forceinline__ void ring_i32x2(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t step_off, int& a,
+                                           int& b) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, step_off, 0);
+  a = (int)v[0];
+  b = (int)v[1];
+}
+
 // Blocks for a grid-stride loop over n items. HIP limits one grid dimension to
 // fewer than 2^32 work-items, and the coefficient arrays of a 33^3 grid already
 // hold more elements than that, so element-wise kernels stride over a capped grid.

@@ -202,16 +202,15 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
     // a uniform step base; only the block's last, partial step masks the pairs
     // past the block's end.  The ring is filled before the f staging, so the
     // two memory latencies of a block overlap.
-    const AS1 char* P = (const AS1 char*)(pairs + B->pbeg * (D + 1));
-    constexpr int64_t kStepBytes = 4 * (D + 1) * 8;
+    const __amdgpu_buffer_rsrc_t P = ring_rsrc(pairs + B->pbeg * (D + 1));
+    constexpr uint32_t kStepBytes = 4 * (D + 1) * 8;
     double ab[kRing][RT];
     int cbits[kRing];
 #define PB_LOAD(slot, step)                                                               \
   do {                                                                                    \
-    const AS1 char* Ps_ = P + kStepBytes * (step);                                        \
-    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                     \
-      ab[slot][rt] = *(const AS1 double*)(Ps_ + loffb[rt]);                               \
-    cbits[slot] = *(const AS1 int*)(Ps_ + coffb);                                         \
+    const uint32_t s_ = kStepBytes * (uint32_t)(step);                                    \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) ab[slot][rt] = ring_f64(P, loffb[rt], s_); \
+    cbits[slot] = ring_i32(P, coffb, s_);                                                 \
   } while (0)
     if (compute) {
 #pragma unroll
