@@ -279,6 +279,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
       // partial step masks the pairs past its end.  The ring is filled before
       // the f staging, so the block's two memory latencies overlap.
       constexpr int kRing = 8;
+      static_assert(4 * kRing + 3 <= kPairPad && 4 * kRing + 3 <= kPadPairs, "the ring reads at most 4 kRing + 3 pairs past a block");
       T ab[kRing][RT];
       int cbits[kRing];
       int qbits[FP32 ? kRing : 1];

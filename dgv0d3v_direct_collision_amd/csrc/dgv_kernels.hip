@@ -145,6 +145,7 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
   constexpr int RT = (D + 15) / 16;
   constexpr int CT = 4 / kPbWaves;
   constexpr int kRing = 8;
+  static_assert(4 * kRing + 3 <= kPairPad, "the ring reads at most 4 kRing + 3 pairs past a block");
   __shared__ double lfa[D * 64], lfb[D * 64];
   __shared__ double lga[MIX ? D * 64 : 1], lgb[MIX ? D * 64 : 1];
   const WorkItem it = items[blockIdx.x];
