@@ -83,7 +83,8 @@ class TrainingSetWriter:
         self.batch, self.seed_base, self.tile = min(int(batch), self.nrows), int(seed_base), tile
         os.makedirs(path, exist_ok=True)
 
-    def run(self, log=None) -> dict:
+    def run(self, log=None, log_every: int = 1) -> dict:
+        """Writes the set; `log(msg)` gets a progress line every `log_every` chunks."""
         from . import DGV_FP64, DGVBatch
         from numpy.lib.format import open_memmap
         g, N, B = self.grid, self.grid.nnodes, self.batch
@@ -119,7 +120,7 @@ class TrainingSetWriter:
                     t_gpu += time.perf_counter() - tg
                     fp[r0:r0 + k] = F
                     qp[r0:r0 + k] = Q
-                    if log:
+                    if log and ((r0 // B) % max(1, log_every) == 0 or r0 + k == self.nrows):
                         log(f"dataset: {r0 + k}/{self.nrows} rows")
         finally:
             b.close()
