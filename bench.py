@@ -100,8 +100,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; the evaluator has no multi-GPU path "
-              "(how it would shard is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the evaluator runs on one GPU and has no multi-GPU path; run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import dgv0d3v_direct_collision_amd as dgv
