@@ -374,7 +374,9 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   do {                                                                                        \
     _Pragma("unroll") for (int u = 0; u < kRing; ++u) {                                       \
       B_ISSUE_BYTES(cbits[(u + 1) % kRing], qbits[(u + 1) % kRing]);   /* next step */         \
+      __builtin_amdgcn_sched_barrier(0);   /* its LDS reads ahead of the products */         \
       STEP_MFMA(u, u == 0);   /* each round starts a fresh matrix-core sum */                 \
+      __builtin_amdgcn_sched_barrier(0);   /* and the wait for them behind */                \
       LOAD32(u, (st_) + u + kRing);                                                           \
       FINISH32();                                                                             \
     }                                                                                         \
@@ -430,7 +432,9 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 #pragma unroll
           for (int u = 0; u < kRing; ++u) {
             ISSUE64((u + 1) % kRing);
+            __builtin_amdgcn_sched_barrier(0);   // LDS reads ahead of the products,
             STEP_MFMA(u, false);
+            __builtin_amdgcn_sched_barrier(0);   // and the wait for them behind
             LOAD64(u, st + u + kRing);
             FINISH64();
           }

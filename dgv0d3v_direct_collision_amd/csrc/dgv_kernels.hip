@@ -288,7 +288,12 @@ __global__ __launch_bounds__(64 * kPbWaves) void k_pb_mfma(const WorkItem* __res
 #pragma unroll
       for (int u = 0; u < kRing; ++u) {
         PB_ISSUE((u + 1) % kRing);
+        // keep the next step's LDS reads ahead of this step's products, and
+        // the wait for them behind (the scheduler would otherwise sink the reads
+        // below the products and wait at once)
+        __builtin_amdgcn_sched_barrier(0);
         PB_MFMA(u);
+        __builtin_amdgcn_sched_barrier(0);
         PB_LOAD(u, st + u + kRing);
         PB_FINISH();
       }
