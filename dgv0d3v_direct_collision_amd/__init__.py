@@ -9,7 +9,7 @@ from ._lib import (DGV_FP32_COMP, DGV_FP64, DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOC
                    DGV_KERNEL_MFMA, DGV_KERNEL_RECORD, DGVError, LIB_PATH, load)
 from .collision import (DGV, DGVBatch, agen_pair, chunk_name, grid_desc, grid_geometry,  # noqa: F401
                         log_file_name, read_sol_file, sol_file_name, write_sol_file)
-from .driver_io import L1Track, MacroLog  # noqa: F401
+from .driver_io import L1Track, MacroLog, MomDerivLog  # noqa: F401
 from .surrogate import Surrogate  # noqa: F401
 
 __all__ = ["DGV", "DGVBatch", "Surrogate", "DGVError", "agen_pair", "grid_geometry", "chunk_name", "grid_desc", "load", "LIB_PATH",

@@ -435,6 +435,13 @@ class DGV:
         self._chk(self._L.dgv_moments_high(self._ctx, dptr(f), self.nnodes, dptr(out)))
         return tuple(out)
 
+    def RecordMomDerivRelaxTimes_DGV(self, log, curr_time: float) -> bool:
+        """dgvtools_mod.f90:1589-2023 into a driver_io.MomDerivLog: the moments'
+        derivatives and relaxation times from the stepper history, f1 (the
+        solution of the previous step) and frhs1 (its right-hand side), as the
+        driver calls it after each multistep step (DGVblzmPL.f90:330)."""
+        return log.record(curr_time, self.hist[0], self.frhs1, self)
+
     def WriteSol_DGV(self, path, f, dt: float):
         """readwrite.f90:1756-1796: e_time, dt, rkmts, f and the multistep
         history (hist, with frhs1) to a Fortran-unformatted file."""

@@ -131,3 +131,117 @@ def test_file_names():
         "sol/run2kc3su3sv3sw3NXU9MuUU9MvVU9MwWU_time0.0100000000_sol.dat"
     assert dgv.log_file_name(5, 2, (-1.0, 1.0), "_L1err.txt", "", "b", k_c=1, N_x=1) == \
         "b1kc2su2sv2sw1NXU5MuUU5MvVU5MwWU_L1err.txt"
+
+
+# ---- RecordMomDerivRelaxTimes_DGV / WriteMomDerivRelaxTimes_DGV -----------------------------
+
+def _grid_arrays(G):
+    return G.nodes_u, G.nodes_v, G.nodes_w, G.nodes_gwts
+
+
+def test_moment_derivatives_of_a_scaled_rhs():
+    # frhs1 = c f1 scales every moment: dn = c n, the bulk velocity and the
+    # temperature do not move, and each centred moment's derivative is c times
+    # the moment itself (MassCheckRecHighCMoments of the oracle)
+    from dgv0d3v_direct_collision_amd.driver_io import moment_derivatives
+    G = oracle.Geometry(4, 3, (-3.0, 3.0))
+    f1 = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, 4)
+    c = -0.37
+    d = moment_derivatives(f1, c * f1, *_grid_arrays(G))
+    m = oracle.masscheck_high(G, f1)
+    assert abs(d[0] - c * m[0]) <= 1e-14 * abs(c * m[0])
+    assert np.all(np.abs(d[1:5]) <= 1e-14)
+    assert np.allclose(d[5:20], c * np.asarray(m[5:20]), rtol=1e-12, atol=1e-15)
+
+
+def test_moment_derivatives_against_finite_differences():
+    from dgv0d3v_direct_collision_amd.driver_io import moment_derivatives
+    G = oracle.Geometry(4, 3, (-3.0, 3.0))
+    u, v, w, gw = _grid_arrays(G)
+    f1 = bimodal(u, v, w, 5)
+    r = bimodal(u, v, w, 6) - 0.8 * f1
+    d = moment_derivatives(f1, r, u, v, w, gw)
+    h = 1e-5
+    n = lambda f: np.sum(f * gw)
+    ub = lambda f: np.array([np.sum(f * gw * x) for x in (u, v, w)]) / n(f)
+    assert abs(d[0] - (n(f1 + h * r) - n(f1 - h * r)) / (2 * h)) <= 1e-10 * abs(d[0])
+    assert np.allclose(d[1:4], (ub(f1 + h * r) - ub(f1 - h * r)) / (2 * h), rtol=1e-7, atol=1e-12)
+    # symmetric f1 and an even rhs: the bulk velocity stays 0 and dT is the
+    # derivative of the temperature
+    def temp(f):
+        b = ub(f)
+        return np.sum(f * gw * ((u - b[0]) ** 2 + (v - b[1]) ** 2 + (w - b[2]) ** 2)) / n(f) * 2.0 / 3.0
+    fs = np.exp(-(u ** 2 + v ** 2 + w ** 2) / 0.7) + 0.3 * np.exp(-(u ** 2 + v ** 2 + w ** 2) / 2.1)
+    rs = np.exp(-(u ** 2 + v ** 2 + w ** 2) / 1.4) - 0.5 * fs
+    ds = moment_derivatives(fs, rs, u, v, w, gw)
+    assert np.all(np.abs(ds[1:4]) <= 1e-13)
+    assert abs(ds[4] - (temp(fs + h * rs) - temp(fs - h * rs)) / (2 * h)) <= 1e-7 * abs(ds[4])
+
+
+def test_relaxation_times_of_a_bgk_rhs():
+    # frhs1 = -(f1 - f_M)/tau with f_M the local Maxwellian of f1: every
+    # relaxation time of the directional temperatures and the centred moments
+    # is 1/tau (the moments of f_M are the Gaussian ones to quadrature
+    # accuracy), and the density's is dn/mn = -1/tau
+    from dgv0d3v_direct_collision_amd.driver_io import moment_derivatives
+    G = oracle.Geometry(7, 6, (-6.0, 6.0))
+    u, v, w, gw = _grid_arrays(G)
+    f1 = bimodal(u, v, w, 7)
+    n, ub, vb, wb, T = oracle.masscheck(G, f1)
+    fm = n * np.exp(-((u - ub) ** 2 + (v - vb) ** 2 + (w - wb) ** 2) / T) / (np.pi * T) ** 1.5
+    tau = 0.4
+    d = moment_derivatives(f1, -(f1 - fm) / tau, u, v, w, gw)
+    t = d[20:]
+    # mn is itself a quadrature error (the Maxwellian's density defect), so
+    # the density's time carries the rounding of the temperature sums
+    assert abs(t[0] + 1 / tau) <= 1e-4 / tau
+    assert np.allclose(t[5:8], 1 / tau, rtol=1e-4)                         # T_u, T_v, T_w
+    # mom3 .. mom6: the discrete Maxwellian's centred moments are Gaussian to
+    # about 1e-4 of these; a wrong constant or sign would be off by O(1)
+    assert np.allclose(t[8:20], 1 / tau, rtol=1e-3)
+
+
+def test_moment_derivative_store():
+    from dgv0d3v_direct_collision_amd.driver_io import MomDerivLog
+    G = oracle.Geometry(3, 3, (-1.5, 1.5))
+    f1 = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, 1)
+    log = MomDerivLog("unused", num_eval_error=1, rkmts=1, t_R=1.0)
+    assert log.capacity == 6
+    assert all(log.record(0.1 * k, f1, -f1, G) for k in range(6))
+    assert not log.record(0.7, f1, -f1, G)                 # full: dropped
+    assert len(log.rows) == 6 and log.rows[-1][0] == 0.5
+    log.record(1.5, f1, -f1, G)                            # past t_R: released
+    with pytest.raises(RuntimeError):
+        log.write()
+
+
+@needs_fc
+def test_moment_derivative_files_match_fortran_formatting(tmp_path):
+    from dgv0d3v_direct_collision_amd.driver_io import MomDerivLog
+    G = oracle.Geometry(3, 3, (-1.5, 1.5))
+    f1 = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, 2)
+    r = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, 3) - f1
+    log = MomDerivLog(str(tmp_path / "run"), num_eval_error=4, rkmts=2, t_R=10.0)
+    log.record(0.25, f1, r, G)
+    log.record(0.5, f1, 2 * r, G)
+    log.write()
+    rows = np.array(log.rows)
+    vals = list(rows[0, :21]) + [0.0, -1.0e-120, 6.02e23, -9.9999996, 1e100] + [1.0] * 16
+    fmt = "(" + "".join(["E18.6, A2, "] * 20) + "E18.6)"
+    fmts = " &\n      ".join(f'"{fmt[i:i + 60]}"//' for i in range(0, len(fmt), 60))[:-2]
+    items = lambda vs: ", &\n    ".join(f'{float(x)!r}_DP, ","' for x in vs[:-1]) + f", &\n    {float(vs[-1])!r}_DP"
+    prog = ("program e\n  implicit none\n  integer, parameter :: DP = kind(1.0d0)\n"
+            "  character(len=*), parameter :: fm = &\n      " + fmts + "\n"
+            "  open (16, file=\"e.txt\", form=\"FORMATTED\")\n"
+            f"  write (16, fm) &\n    {items(vals[:21])}\n"
+            f"  write (16, fm) &\n    {items(vals[21:])}\n"
+            "  close (16)\nend program e\n")
+    (tmp_path / "e.f90").write_text(prog)
+    subprocess.run([FC, "-O0", "e.f90", "-o", "e"], cwd=tmp_path, check=True)
+    subprocess.run(["./e"], cwd=tmp_path, check=True)
+    ref = (tmp_path / "e.txt").read_text().splitlines()
+    from dgv0d3v_direct_collision_amd.driver_io import fortran_e
+    assert (tmp_path / "run_dermom.txt").read_text().splitlines()[2] == ref[0]
+    assert " ,".join(fortran_e(x, 18, 6) for x in vals[21:]) == ref[1]
+    rel = (tmp_path / "run_relaxtimes.txt").read_text().splitlines()
+    assert len(rel) == 4 and rel[2].split(",")[0].strip() == "0.250000E+00"

@@ -132,3 +132,27 @@ def test_high_moments_match_oracle(env):
     got = np.array(d.MassCheckRecHighCMoments(f))
     assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(np.abs(ref), 1e-3))
     assert np.allclose(got[:5], d.MassCheckRec(f), rtol=1e-14, atol=0)
+
+
+def test_moment_derivative_record_reads_the_step_history(env):
+    # after a multistep step, f1 is the solution the step started from and
+    # frhs1 its right-hand side (DGVblzmPL.f90:326-330); the record made from
+    # the device stepper's history equals the one made from the oracle's step
+    from dgv0d3v_direct_collision_amd.driver_io import MomDerivLog, moment_derivatives
+    G, A, d, f, q = env
+    n = len(f)
+    rng = np.random.default_rng(11)
+    hist = np.zeros((9, n))
+    hist[:4] = f * (1 + 0.01 * rng.standard_normal((4, n)))
+    hist[4:] = 1e-3 * rng.standard_normal((5, n))
+    fr, hr = oracle.ab_step(q, f, hist, 0.02, 1.0, 3)
+    d.rkmts, d.coef = 3, 1.0
+    d.hist = hist.copy()
+    d.frhs1 = hist[4].copy()
+    fg = f.copy()
+    d.TimeIntegratorMTS_SH_DGV(fg, 0.02)
+    log = MomDerivLog("unused", num_eval_error=10, rkmts=3, t_R=1.0)
+    assert d.RecordMomDerivRelaxTimes_DGV(log, 0.0)
+    ref = moment_derivatives(hr[0], hr[4], G.nodes_u, G.nodes_v, G.nodes_w, G.nodes_gwts)
+    assert np.array_equal(hr[0], f)
+    assert np.allclose(log.rows[0][1:21], ref[:20], rtol=1e-9, atol=1e-14)
