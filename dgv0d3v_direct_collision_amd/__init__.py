@@ -8,7 +8,8 @@ host-side mirror of the reference's procedures (see ``collision.DGV``).
 from ._lib import (DGV_FP32_COMP, DGV_FP64, DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK,  # noqa: F401
                    DGV_KERNEL_MFMA, DGV_KERNEL_RECORD, DGVError, LIB_PATH, load)
 from .collision import (DGV, DGVBatch, agen_pair, chunk_name, grid_desc, grid_geometry,  # noqa: F401
-                        log_file_name, read_sol_file, sol_file_name, write_sol_file)
+                        log_file_name, read_sol_file, sol_file_name, write_sol_file,
+                        write_sol_init_file)
 from .driver_io import L1Track, MacroLog, MomDerivLog  # noqa: F401
 from .surrogate import Surrogate  # noqa: F401
 

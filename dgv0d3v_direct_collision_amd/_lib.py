@@ -102,6 +102,7 @@ SIGNATURES = {
     "dgv_sol_file_name": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _i32, _i32, C.c_char_p, C.c_char_p, C.c_size_t]),
     "dgv_log_file_name": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _i32, _i32, C.c_char_p, C.c_char_p, C.c_size_t]),
     "dgv_sol_write_file": (C.c_int, [C.c_char_p, _dbl, _dbl, _i32, _i64, _dp, _dp]),
+    "dgv_sol_write_init_file": (C.c_int, [C.c_char_p, _dbl, _dbl, _i32, _i64, _dp]),
     "dgv_sol_read_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int32), _i64, _dp, _dp]),
     "dgv_collision_moments": (C.c_int, [_vp, _dp, _i64, _dp]),

@@ -108,6 +108,12 @@ def write_sol_file(path, e_time, dt, rkmts, f, hist=None):
                                     None if h is None else dptr(h)))
 
 
+def write_sol_init_file(path, e_time, dt, rkmts, f):
+    """WriteSol_init_DGV (readwrite.f90:1810-1850): the solution file with f alone."""
+    f = np.ascontiguousarray(f, np.float64)
+    check(load().dgv_sol_write_init_file(str(path).encode(), e_time, dt, rkmts, len(f), dptr(f)))
+
+
 def read_sol_file(path, n):
     """RestartSolutionDGV (readwrite.f90:2389-2513) -> (e_time, dt, rkmts, f, hist)."""
     f = np.zeros(n)
@@ -447,6 +453,11 @@ class DGV:
         history (hist, with frhs1) to a Fortran-unformatted file."""
         self.hist[4] = self.frhs1
         write_sol_file(path, self.e_time, dt, self.rkmts, self._f(f), self.hist)
+
+    def WriteSol_init_DGV(self, path, f, dt: float):
+        """readwrite.f90:1810-1850: e_time, dt, rkmts and f alone (the driver's
+        initial-data file)."""
+        write_sol_init_file(path, self.e_time, dt, self.rkmts, self._f(f))
 
     def RestartSolutionDGV(self, path):
         """readwrite.f90:2389-2513: restores e_time, rkmts and the history;

@@ -2,6 +2,7 @@
 //
 //   MakeBaseNameDGV       readwrite.f90:1505-1560   experiment base name (uniform meshes)
 //   WriteSol_DGV          readwrite.f90:1756-1796   <base>_time<suff>_sol.dat
+//   WriteSol_init_DGV     readwrite.f90:1810-1850   the same file with f alone
 //   RestartSolutionDGV    readwrite.f90:2389-2513   reads it back
 //
 // The file is Fortran unformatted sequential: every record is framed by its
@@ -102,6 +103,25 @@ int dgv_sol_write_file(const char* path, double e_time, double dt, int32_t rkmts
   for (int k = 0; k < rkmts - 1; ++k) sol.push_back({hist + (size_t)(4 + k) * n, nb});  // frhs1..
   if (!put_record(o, {{td, 16}}) || !put_record(o, {{&rkmts, 4}}) || !put_record(o, {{&m, 4}}) ||
       !put_record(o, sol)) {
+    set_global_error(std::string("write failed (or a record above 2 GiB): ") + path);
+    return DGV_ERR_IO;
+  }
+  return DGV_OK;
+}
+
+int dgv_sol_write_init_file(const char* path, double e_time, double dt, int32_t rkmts, int64_t n,
+                            const double* f) {
+  if (!path || !f || n < 0 || n > INT32_MAX) { set_global_error("bad argument"); return DGV_ERR_ARG; }
+  if (rkmts < 1 || rkmts > 5) {
+    set_global_error("Unsupported degree of Multiple Time Stepping method passed");
+    return DGV_ERR_ARG;
+  }
+  std::ofstream o(path, std::ios::binary);
+  if (!o) { set_global_error(std::string("cannot open ") + path); return DGV_ERR_IO; }
+  const double td[2] = {e_time, dt};
+  const int32_t m = (int32_t)n;
+  if (!put_record(o, {{td, 16}}) || !put_record(o, {{&rkmts, 4}}) || !put_record(o, {{&m, 4}}) ||
+      !put_record(o, {{f, sizeof(double) * (size_t)n}})) {
     set_global_error(std::string("write failed (or a record above 2 GiB): ") + path);
     return DGV_ERR_IO;
   }

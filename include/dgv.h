@@ -227,6 +227,10 @@ int dgv_log_file_name(const dgv_grid_desc* grid, const char* dir, const char* ba
  * hist = f1..f4, frhs1..frhs5 (9 n doubles, may be NULL for rkmts 1) */
 int dgv_sol_write_file(const char* path, double e_time, double dt, int32_t rkmts, int64_t n,
                        const double* f, const double* hist);
+/* WriteSol_init_DGV (readwrite.f90:1810-1850): the same records with f alone
+ * in the last, whatever rkmts (the driver's initial-data file, suffix "i") */
+int dgv_sol_write_init_file(const char* path, double e_time, double dt, int32_t rkmts, int64_t n,
+                            const double* f);
 /* RestartSolutionDGV (readwrite.f90:2389-2513): the reverse; fails when the
  * stored size differs from n.  rkmts receives the file's order. */
 int dgv_sol_read_file(const char* path, double* e_time, double* dt, int32_t* rkmts, int64_t n,

@@ -4,10 +4,11 @@
    sampled on a fine grid the closed forms hold -- T_d = T/3 (the reference's
    2/3 factor on a one-dimensional variance T/2), odd central moments 0,
    mom4 = 3 (T/2)^2, mom6 = 15 (T/2)^3 -- and T = T_u + T_v + T_w for any f;
- * the solution checkpoint of WriteSol_DGV / RestartSolutionDGV: the library's
-   file is byte for byte the file a Fortran program writes with the
-   reference's write statements (built with amdflang here), reads back for
-   every rkmts, and a size mismatch is refused;
+ * the solution checkpoint of WriteSol_DGV / RestartSolutionDGV and the
+   initial-data file of WriteSol_init_DGV: the library's files are byte for
+   byte the files a Fortran program writes with the reference's write
+   statements (built with amdflang here), read back for every rkmts, and a
+   size mismatch is refused;
  * the file names of MakeBaseNameDGV, and the macro / L1 log lines against the
    same Fortran runtime's formatting of the reference's format strings.
 """
@@ -67,6 +68,12 @@ program w
   write (15) size(f, 1)
   write (15) f, f1, f2, frhs1, frhs2
   close (15)
+  open (17, file="ref_init.dat", position="REWIND", action="WRITE", form="UNFORMATTED", access="SEQUENTIAL")
+  write (17) e_time, dt
+  write (17) rkmts
+  write (17) size(f, 1)
+  write (17) f
+  close (17)
   open (16, file="macro.txt", form="FORMATTED")
   write (16, "(F18.14, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15,  A2, F18.15, " // &
             "A2, F18.14, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2, F18.15, A2," // &
@@ -100,6 +107,11 @@ def test_solution_file_matches_fortran_and_reads_back(tmp_path):
     assert np.array_equal(f2, f) and np.array_equal(h2[[0, 1, 4, 5]], hist[[0, 1, 4, 5]])
     with pytest.raises(dgv.DGVError):
         dgv.read_sol_file(tmp_path / "ref_sol.dat", n + 1)
+    # WriteSol_init_DGV: f alone whatever rkmts
+    dgv.write_sol_init_file(tmp_path / "ours_init.dat", 0.125, 0.0025, 3, f)
+    assert (tmp_path / "ours_init.dat").read_bytes() == (tmp_path / "ref_init.dat").read_bytes()
+    with pytest.raises(dgv.DGVError):
+        dgv.write_sol_init_file(tmp_path / "bad.dat", 0.0, 0.1, 6, f)
     # the log lines against the Fortran runtime's formatting
     lines = (tmp_path / "macro.txt").read_text().splitlines()
     vals = [1.0, 0.1, -0.2, 0.3, 0.75, 0.25, 0.25, 0.25, -0.001, 0.002, 0.0, 0.1875, 0.19, 0.2, -1.0e-5, 3.0e-4,
