@@ -292,6 +292,27 @@ def main():
     flops = 2.0 * V * B   # one multiply-add per in-grid record visit, per VDF
     achieved = flops / (r32["kernel_ms"] * 1e-3) / 1e12
     single_qps = 1e3 / ms_single
+    # the HBM roofline: algorithmic bytes per Q (A records streamed once per
+    # launch: 16 B each in FP64 and 12 B in FP32 with the codes, f read and Q
+    # written once) against 8 TB/s, and the counter-measured traffic
+    bytes_b = 12.0 * R + 16.0 * N * B
+    bytes_1 = 16.0 * R + 16.0 * N
+    tr_b = measured_traffic("k_batch_mfma<27,true>", f"batch{B}_M{M}")
+    tr_1 = measured_traffic("k_pb_mfma<27,false>", f"single_M{M}")
+
+    def hbm(bytes_alg, traffic, kernel_ms, rate, per_unit):
+        t = kernel_ms * 1e-3
+        o = {"algorithmic_gbs": bytes_alg / t / 1e9, "algorithmic_frac": bytes_alg / t / 1e9 / HBM_PEAK_GBS,
+             "rate_bound_at_hbm_peak": HBM_PEAK_GBS * 1e9 / (bytes_alg / per_unit),
+             "rate_frac_of_hbm_roofline": rate / (HBM_PEAK_GBS * 1e9 / (bytes_alg / per_unit))}
+        if traffic:
+            o.update({"measured_gbs": traffic / t / 1e9, "measured_frac": traffic / t / 1e9 / HBM_PEAK_GBS})
+        return o
+    hbm_b = hbm(bytes_b, tr_b, r32["kernel_ms"], value, B)
+    hbm_1 = hbm(bytes_1, tr_1, ms_single_main, single_qps, 1)
+    if rk and rk["projected"].get("steps_per_sec"):
+        hbm_1["rk3_steps_per_s_bound_at_hbm_peak"] = hbm_1["rate_bound_at_hbm_peak"] / 3.0
+        hbm_1["rk3_frac_of_hbm_roofline"] = rk["projected"]["steps_per_sec"] / hbm_1["rk3_steps_per_s_bound_at_hbm_peak"]
     out = {
         "metric": "Q(f) evals/sec at 27^3 nodes: FP64 plain vs batched FP32-comp, rel err <=1e-6",
         "value": value, "unit": "VDF Q(f) evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -312,16 +333,16 @@ def main():
         },
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
-                     "traffic": measured_traffic("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
+                     "traffic": tr_b,
                      "l2": measured_l2("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
                      "kernel": "k_batch_mfma<27,FP32>", "kernel_ms": r32["kernel_ms"],
                      "flops_per_launch": flops,
-                     "hbm_bytes_per_launch_algorithmic": 12.0 * R + 16.0 * N * B,
-                     "fp64_plain": {"kernel": knames[best], "kernel_ms": ms_single_main,
+                     "hbm_bytes_per_launch_algorithmic": bytes_b, "hbm": hbm_b,
+                     "fp64_plain": {"kernel": knames[best], "kernel_ms": ms_single_main, "hbm": hbm_1,
                                     "tflops": 2.0 * V / (ms_single_main * 1e-3) / 1e12,
                                     "frac_fp64_peak": 2.0 * V / (ms_single_main * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                                     "hbm_gbs_algorithmic": (16.0 * R + 16.0 * N) / (ms_single_main * 1e-3) / 1e9,
-                                    "traffic": measured_traffic("k_pb_mfma<27,false>", f"single_M{M}"),
+                                    "traffic": tr_1,
                                     "l2": measured_l2("k_pb_mfma<27,false>", f"single_M{M}")}},
         "cpu_baseline": cpu,
         "conservation": {"sum_w_psi_Q": [float(x) for x in mom], "sum_abs_w_psi_Q": absmom,
