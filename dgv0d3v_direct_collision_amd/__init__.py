@@ -5,7 +5,7 @@ alexmalekseenko/DGV0D3V_direct_collision: hand-written gfx950 HIP kernels behind
 the C ABI of ``include/dgv.h`` (libdgv_amd.so), with this package as the
 host-side mirror of the reference's procedures (see ``collision.DGV``).
 """
-from ._lib import (DGV_FP32_COMP, DGV_FP64, DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK,  # noqa: F401
+from ._lib import (DGV_FP32_COMP, DGV_FP32_F64ACC, DGV_FP64, DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK,  # noqa: F401
                    DGV_KERNEL_MFMA, DGV_KERNEL_RECORD, DGVError, LIB_PATH, load)
 from .collision import (DGV, DGVBatch, agen_pair, chunk_name, grid_desc, grid_geometry,  # noqa: F401
                         log_file_name, read_sol_file, sol_file_name, write_sol_file,

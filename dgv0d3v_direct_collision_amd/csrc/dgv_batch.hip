@@ -189,13 +189,19 @@ __device__ __forceinline__ int f32_row_fn(int rt, int i) {
   return min(l, D);
 }
 
-template <int D, bool FP32>
+// MODE is the batch's precision: DGV_FP64, DGV_FP32_COMP (FP32 products and
+// matrix-core sums, two-sum compensated FP32 accumulation of the 32-pair
+// sums) or DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in
+// FP64 instead: two VALU instructions per element against seven).
+template <int D, int MODE>
 __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
                                                                    const BatchGridDev* __restrict__ grids,
                                                                    const double* __restrict__ f64,
                                                                    const float* __restrict__ f32,
                                                                    double* __restrict__ partial,
                                                                    int* __restrict__ counter) {
+  constexpr bool FP32 = MODE != DGV_FP64;
+  constexpr bool ACC64 = MODE == DGV_FP32_F64ACC;
   constexpr int RT = (D + 15) / 16;
   constexpr int CH = chunks_per_item<FP32>();    // chunks per item
   constexpr int WPC = kWavesPerItem / CH;        // waves per chunk
@@ -239,6 +245,7 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
     const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
     const int64_t vb = (int64_t)(it.vc0 + ch) * 64 + lane;   // the instance this thread stages
     V4 acc[RT][CT], hi[RT][CT], lo[RT][FP32 ? CT : 1];
+    bd4 hd[RT][ACC64 ? CT : 1];   // DGV_FP32_F64ACC: the FP64 running sums
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -246,22 +253,28 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
         acc[rt][ct] = V4{0, 0, 0, 0};
         hi[rt][ct] = V4{0, 0, 0, 0};
         if constexpr (FP32) lo[rt][ct] = V4{0, 0, 0, 0};
+        if constexpr (ACC64) hd[rt][ct] = bd4{0, 0, 0, 0};
       }
     T* la = lfa[ch];
     T* lb = lfb[ch];
     int prev_xc = -1;   // the cell whose f is in la (blocks come ordered by a)
-    // fold a round's matrix-core sums into the FP32 (hi, lo) pair (the next
-    // round's first products start a fresh sum, so acc needs no clearing);
-    // element 3 of tile 1 holds only unused rows (f32_row_fn) and is skipped
+    // fold a round's matrix-core sums into the FP32 (hi, lo) pair, or into the
+    // FP64 sums (the next round's first products start a fresh sum, so acc
+    // needs no clearing); element 3 of tile 1 holds only unused rows
+    // (f32_row_fn) and is skipped
 #define FOLD()                                                                                \
   do {                                                                                        \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt)                                         \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                         \
     _Pragma("unroll") for (int e = 0; e < (rt == 0 ? 4 : 3); ++e) {                           \
-      float s2_, er_;                                                                         \
-      two_sum(hi[rt][ct][e], acc[rt][ct][e], s2_, er_);                                       \
-      hi[rt][ct][e] = s2_;                                                                    \
-      lo[rt][ct][e] += er_;                                                                   \
+      if constexpr (ACC64) {                                                                  \
+        hd[rt][ct][e] += (double)acc[rt][ct][e];                                              \
+      } else {                                                                                \
+        float s2_, er_;                                                                       \
+        two_sum(hi[rt][ct][e], acc[rt][ct][e], s2_, er_);                                     \
+        hi[rt][ct][e] = s2_;                                                                  \
+        lo[rt][ct][e] += er_;                                                                 \
+      }                                                                                       \
     }                                                                                         \
   } while (0)
     for (int bi = it.b0; bi < it.b1; ++bi) {
@@ -482,7 +495,8 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
           // output layouts: FP64 row 4e + lane/16, FP32 row 4 (lane/16) + e
           const int l = FP32 ? f32_row_fn<D>(rt, 4 * kk + e) : 16 * rt + 4 * e + kk;
           if (l < D) {
-            if constexpr (FP32) out[l] = (double)hi[rt][ct][e] + (double)lo[rt][ct][e];
+            if constexpr (ACC64) out[l] = hd[rt][ct][e];
+            else if constexpr (FP32) out[l] = (double)hi[rt][ct][e] + (double)lo[rt][ct][e];
             else out[l] = hi[rt][ct][e];
           }
         }
@@ -688,7 +702,7 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
           acc = 0;
         }
       }
-      const int chpi = b->precision == DGV_FP32_COMP ? chunks_per_item<true>() : chunks_per_item<false>();
+      const int chpi = b->precision != DGV_FP64 ? chunks_per_item<true>() : chunks_per_item<false>();
       for (int vc0 = 0; vc0 < nchunk; vc0 += chpi) {
         const int nvc = std::min(chpi, nchunk - vc0);
         std::vector<int64_t> first;   // first partial row of each segment's item
@@ -755,30 +769,27 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
   const int D = b->grids[0]->g.D;
   const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
-  if (b->precision == DGV_FP32_COMP)
+  if (b->precision != DGV_FP64)
     hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(b->total, 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
   // workgroups of four waves; waves_per_cu sets the persistent grid's size
   const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
-  const bool fp32 = b->precision == DGV_FP32_COMP;
   const unsigned nwg = (unsigned)std::max(1, std::min(nitems, b->ncu * wpc / kWavesPerItem));
   if (nitems > 0) {
     const dim3 blk(64 * kWavesPerItem);
+#define DGV_BATCH_LAUNCH(DD, MM)                                                                          \
+  hipLaunchKernelGGL((k_batch_mfma<DD, MM>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f, \
+                     b->d_f32, P.d_partial, b->d_counter)
     if (D == 27) {
-      if (fp32)
-        hipLaunchKernelGGL((k_batch_mfma<27, true>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
-      else
-        hipLaunchKernelGGL((k_batch_mfma<27, false>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+      if (b->precision == DGV_FP32_COMP) DGV_BATCH_LAUNCH(27, DGV_FP32_COMP);
+      else if (b->precision == DGV_FP32_F64ACC) DGV_BATCH_LAUNCH(27, DGV_FP32_F64ACC);
+      else DGV_BATCH_LAUNCH(27, DGV_FP64);
     } else {
-      if (fp32)
-        hipLaunchKernelGGL((k_batch_mfma<8, true>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
-      else
-        hipLaunchKernelGGL((k_batch_mfma<8, false>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f,
-                           b->d_f32, P.d_partial, b->d_counter);
+      if (b->precision == DGV_FP32_COMP) DGV_BATCH_LAUNCH(8, DGV_FP32_COMP);
+      else if (b->precision == DGV_FP32_F64ACC) DGV_BATCH_LAUNCH(8, DGV_FP32_F64ACC);
+      else DGV_BATCH_LAUNCH(8, DGV_FP64);
     }
+#undef DGV_BATCH_LAUNCH
   }
   if (reduce && P.nred > 0) {
     if (D == 27)
@@ -857,8 +868,8 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     return DGV_ERR_ARG;
   }
   *out = nullptr;
-  if (precision != DGV_FP64 && precision != DGV_FP32_COMP) {
-    set_global_error("precision must be DGV_FP64 or DGV_FP32_COMP");
+  if (precision != DGV_FP64 && precision != DGV_FP32_COMP && precision != DGV_FP32_F64ACC) {
+    set_global_error("precision must be DGV_FP64, DGV_FP32_COMP or DGV_FP32_F64ACC");
     return DGV_ERR_ARG;
   }
   const int D = grids[0] ? grids[0]->g.D : 0;
@@ -905,7 +916,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     if (hipMalloc(&b->d_tmp[i], nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
   if (hipMemsetAsync(b->d_f, 0, nb, b->stream) || hipMemsetAsync(b->d_q, 0, nb, b->stream))
     return fail(DGV_ERR_HIP, "hipMemset");
-  if (precision == DGV_FP32_COMP && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
+  if (precision != DGV_FP64 && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
     return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
   if (hipMalloc(&b->d_counter, sizeof(int) * 4)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
   {
@@ -956,7 +967,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
         hipMemcpy(b->d_blist[g], blist.data(), sizeof(int32_t) * blist.size(), hipMemcpyHostToDevice))
       return fail(DGV_ERR_NOMEM, "block lists");
     b->h_blist[g] = blist;
-    if (precision == DGV_FP32_COMP) {
+    if (precision != DGV_FP64) {
       // zero pairs past the end: the kernel's load ring reads up to 4 kRing + 3
       // pairs beyond a block without clamping
       const int64_t n = pb.npairs * (D + 2), pad = (int64_t)kPadPairs * (D + 2);

@@ -36,7 +36,7 @@ enum {
   DGV_ERR_UNSUPPORTED = -8
 };
 
-enum { DGV_FP64 = 0, DGV_FP32_COMP = 1 };                 /* precision       */
+enum { DGV_FP64 = 0, DGV_FP32_COMP = 1, DGV_FP32_F64ACC = 2 };   /* precision */
 enum { DGV_KERNEL_AUTO = 0, DGV_KERNEL_PAIRBLOCK = 1, DGV_KERNEL_RECORD = 2, DGV_KERNEL_MFMA = 3 };
 
 typedef struct dgv_ctx dgv_ctx;
@@ -249,6 +249,9 @@ int dgv_project_invariants_dev(dgv_ctx* ctx, double* d_q, int64_t n);
  * instance is one VDF on one of the given grid contexts (each with its A-array).
  * f of all instances is packed per grid in the [cell][local][instance] order. */
 typedef struct dgv_batch dgv_batch;
+/* precision: DGV_FP64; DGV_FP32_COMP (FP32 products and matrix-core sums over
+ * 32 pairs, two-sum compensated FP32 accumulation, FP64 final sums);
+ * DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in FP64) */
 int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_of_instance,
                      int32_t ninstances, int32_t precision, dgv_batch** out);
 void dgv_batch_destroy(dgv_batch* b);

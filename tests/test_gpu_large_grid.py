@@ -6,8 +6,9 @@ grid dimension to fewer than 2^32 work-items, so a one-thread-per-element launch
 would silently cover only the first part. Checked at full size through
 size-independent properties:
  * FP64 batch == single-VDF evaluator (relative L2 <= 1e-12);
- * FP32-compensated batch against FP64, per node, within 4 * 2^-24 of the
-   term scale 2/w sum |A| f f (its FP32 coefficients are converted element-wise);
+ * both FP32 batches (two-sum compensated, and FP64-accumulated) against FP64,
+   per node, within 4 * 2^-24 of the term scale 2/w sum |A| f f (their FP32
+   coefficients are converted element-wise);
  * the |A| context (element-wise fabs) obeys the triangle inequality
    |Q(f)| <= Q_|A|(f) at every node for f >= 0.
 """
@@ -36,7 +37,7 @@ def test_33cubed_grid_past_2_32_coefficients():
         f += n * np.exp(-((u - ub[0]) ** 2 + (v - ub[1]) ** 2 + (w - ub[2]) ** 2) / T) / (np.pi * T) ** 1.5
     q_single = g.EvalCollisionPeriodicAPlus_DGV(f)
     qs = {}
-    for prec in ("FP64", "FP32_COMP"):
+    for prec in ("FP64", "FP32_COMP", "FP32_F64ACC"):
         b = dgv.DGVBatch([g], np.zeros(4, np.int32), getattr(dgv, "DGV_" + prec))
         for i in range(4):
             b.set_f(i, f)
@@ -54,9 +55,10 @@ def test_33cubed_grid_past_2_32_coefficients():
     slack = np.min(qabs - np.abs(q_single)) / np.max(qabs)
     print(f"[large] triangle slack min {slack:.3e}", flush=True)
     assert slack >= -1e-12
-    for i in range(4):
-        err = np.abs(qs["FP32_COMP"][i] - qs["FP64"][i])
-        ratio = np.max(err / np.maximum(qabs, 1e-300)) / 2.0 ** -24
-        print(f"[large] FP32 instance {i}: rel L2 {rel_l2(qs['FP32_COMP'][i], qs['FP64'][i]):.3e}, "
-              f"max error / term scale {ratio:.3f} * 2^-24", flush=True)
-        assert ratio <= 4.0
+    for prec in ("FP32_COMP", "FP32_F64ACC"):
+        for i in range(4):
+            err = np.abs(qs[prec][i] - qs["FP64"][i])
+            ratio = np.max(err / np.maximum(qabs, 1e-300)) / 2.0 ** -24
+            print(f"[large] {prec} instance {i}: rel L2 {rel_l2(qs[prec][i], qs['FP64'][i]):.3e}, "
+                  f"max error / term scale {ratio:.3f} * 2^-24", flush=True)
+            assert ratio <= 4.0
