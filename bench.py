@@ -165,7 +165,9 @@ def main():
     Q_rows = np.zeros_like(F_rows)
     results = {}
     qb = {}
-    for prec in (["FP32_COMP"] if args.no_fp64_batch or args.profile else ["FP32_COMP", "FP64"]):
+    # FP32_COMP is the headline (two-sum compensated FP32 sums); FP32_F64ACC
+    # accumulates the same FP32 matrix-core sums in FP64
+    for prec in (["FP32_COMP"] if args.no_fp64_batch or args.profile else ["FP32_COMP", "FP32_F64ACC", "FP64"]):
         b = dgv.DGVBatch([d], which, getattr(dgv, "DGV_" + prec))
         b.set_f_rows(d_rows, 0, B)
         if prec == "FP32_COMP" and args.tile <= 0:
@@ -217,12 +219,20 @@ def main():
         dabs = dgv.DGV(M, 3, (-L, L))
         dabs.generate_aarray(min_sens=0.0, nquad=args.nquad)
         dabs.aarray_abs()
-        kap, sc = [], []
+        kap, sc, sc64 = [], [], []
         for i in range(B):
             qa = dabs.EvalCollisionPeriodicAPlus_DGV(fs[i])
             kap.append(float(np.linalg.norm(qa) / np.linalg.norm(qb["FP64"][i])))
             sc.append(float(np.linalg.norm(qb["FP32_COMP"][i] - qb["FP64"][i]) / np.linalg.norm(qa)))
+            sc64.append(float(np.linalg.norm(qb["FP32_F64ACC"][i] - qb["FP64"][i]) / np.linalg.norm(qa)))
         dabs.close()
+        e64 = [rel_l2(qb["FP32_F64ACC"][i], qb["FP64"][i]) for i in range(B)]
+        results["FP32_F64ACC"]["error_vs_fp64"] = {
+            "rel_l2_max": max(e64), "rel_l2_median": float(np.median(e64)),
+            "frac_vdfs_within_1e-6": float(np.mean(np.array(e64) <= 1e-6)),
+            "err_over_term_scale_in_2^-24": max(sc64) * 2.0 ** 24}
+        log(f"FP32 + FP64 accumulation vs FP64: rel L2 max {max(e64):.3e}, error / term scale "
+            f"{max(sc64) * 2 ** 24:.2f} x 2^-24")
         err32.update({"kappa_median": float(np.median(kap)), "kappa_max": max(kap),
                       "err_over_term_scale_max": max(sc), "err_over_term_scale_in_2^-24": max(sc) * 2.0 ** 24})
         log(f"conditioning ||sum|A f f| || / ||Q||: median {np.median(kap):.1f}, max {max(kap):.1f}; "
