@@ -307,7 +307,7 @@ def main():
     # written once) against 8 TB/s, and the counter-measured traffic
     bytes_b = 12.0 * R + 16.0 * N * B
     bytes_1 = 16.0 * R + 16.0 * N
-    tr_b = measured_traffic("k_batch_mfma<27,true>", f"batch{B}_M{M}")
+    tr_b = measured_traffic("k_batch_mfma<27,1>", f"batch{B}_M{M}")
     tr_1 = measured_traffic("k_pb_mfma<27,false>", f"single_M{M}")
 
     def hbm(bytes_alg, traffic, kernel_ms, rate, per_unit):
@@ -344,7 +344,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
                      "traffic": tr_b,
-                     "l2": measured_l2("k_batch_mfma<27,true>", f"batch{B}_M{M}"),
+                     "l2": measured_l2("k_batch_mfma<27,1>", f"batch{B}_M{M}"),
                      "kernel": "k_batch_mfma<27,FP32>", "kernel_ms": r32["kernel_ms"],
                      "flops_per_launch": flops,
                      "hbm_bytes_per_launch_algorithmic": bytes_b, "hbm": hbm_b,
