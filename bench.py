@@ -374,8 +374,9 @@ def main():
 def mixed_grid_leg(dgv, d27, rng, args):
     """512 VDFs of uneven size in one persistent launch: 128 on 15^3 (5 cells,
     [-3.5,3.5]^3), 128 on 21^3 (7 cells, [-4,4]^3), 256 on 27^3 (the main grid);
-    seeded 1-3 Maxwellian mixtures; FP32-compensated and FP64 timed; a
-    hipGraph-captured RK2 step of the whole batch."""
+    seeded 1-3 Maxwellian mixtures; FP32-compensated, FP32 with FP64
+    accumulation and FP64 timed; a hipGraph-captured RK2 step of the whole
+    batch."""
     log("mixed grids: A-arrays for 15^3 and 21^3")
     g15 = dgv.DGV(5, 3, (-3.5, 3.5))
     g21 = dgv.DGV(7, 3, (-4.0, 4.0))
@@ -387,7 +388,7 @@ def mixed_grid_leg(dgv, d27, rng, args):
     fs = [mixture_vdf(g.nodes_u, g.nodes_v, g.nodes_w, g.nodes_gwts, rng, kmax=3) for g in (grids[k] for k in which)]
     out, qs = {}, {}
     visits = sum(grids[k].workload()["visits"] for k in which)
-    for prec in ("FP32_COMP", "FP64"):
+    for prec in ("FP32_COMP", "FP32_F64ACC", "FP64"):
         b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
@@ -408,6 +409,8 @@ def mixed_grid_leg(dgv, d27, rng, args):
     e = np.array([rel_l2(qs["FP32_COMP"][i], qs["FP64"][i]) for i in range(len(which))])
     out["fp32comp_error_vs_fp64"] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
                                      "frac_vdfs_within_1e-6": float(np.mean(e <= 1e-6))}
+    e64 = np.array([rel_l2(qs["FP32_F64ACC"][i], qs["FP64"][i]) for i in range(len(which))])
+    out["fp32_f64acc_error_vs_fp64"] = {"rel_l2_max": float(e64.max()), "rel_l2_median": float(np.median(e64))}
     out["grids"] = {"15^3": 128, "21^3": 128, "27^3": 256}
     g15.close()
     g21.close()
@@ -417,7 +420,7 @@ def mixed_grid_leg(dgv, d27, rng, args):
 def precision_leg(dgv, rng, args):
     """64 near-equilibrium VDFs on 33^3 (11 cells on [-4.5,4.5]^3): a Maxwellian
     plus a seeded perturbation of relative amplitude 1e-3, 1e-2 or 1e-1;
-    FP32-compensated against the FP64 path (FP64 is held to the oracle by the
+    FP32-compensated (and FP32 with FP64 accumulation) against the FP64 path (FP64 is held to the oracle by the
     parity tests); relative L2 and max-norm errors per VDF, the conditioning
     of each Q, and the moments after the FP64 projection."""
     M = 11
@@ -436,7 +439,7 @@ def precision_leg(dgv, rng, args):
         amp_of.append(a)
     which = np.zeros(64, np.int32)
     qs, times = {}, {}
-    for prec in ("FP32_COMP", "FP64"):
+    for prec in ("FP32_COMP", "FP32_F64ACC", "FP64"):
         b = dgv.DGVBatch([g], which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
@@ -460,13 +463,16 @@ def precision_leg(dgv, rng, args):
         em = [float(np.max(np.abs(qs["FP32_COMP"][i] - qs["FP64"][i])) / np.max(np.abs(qs["FP64"][i]))) for i in idx]
         kap = [float(np.linalg.norm(g.EvalCollisionPeriodicAPlus_DGV(fs[i])) / np.linalg.norm(qs["FP64"][i]))
                for i in idx]
+        e64 = [rel_l2(qs["FP32_F64ACC"][i], qs["FP64"][i]) for i in idx]
         per[f"{a:g}"] = {"vdfs": len(idx), "rel_l2_max": max(e), "rel_l2_median": float(np.median(e)),
-                         "rel_max_norm_max": max(em), "kappa_median": float(np.median(kap))}
+                         "rel_max_norm_max": max(em), "kappa_median": float(np.median(kap)),
+                         "fp32_f64acc_rel_l2_max": max(e64)}
         log(f"precision 33^3, amplitude {a:g}: FP32-comp rel L2 max {max(e):.2e}, median {np.median(e):.2e}, "
             f"kappa median {np.median(kap):.0f}")
     g.close()
     return {"nodes": 35937, "vdfs": 64, "per_perturbation_amplitude": per,
             "fp32comp_vdf_q_per_s": 64 / times["FP32_COMP"], "fp64_vdf_q_per_s": 64 / times["FP64"],
+            "fp32_f64acc_vdf_q_per_s": 64 / times["FP32_F64ACC"],
             "projected_moment_rel_max": proj}
 
 
