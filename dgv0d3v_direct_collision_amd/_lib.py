@@ -17,6 +17,7 @@ DGV_OK = 0
 ERRORS = {-1: "DGV_ERR_ARG", -2: "DGV_ERR_GRID", -3: "DGV_ERR_INDEX", -4: "DGV_ERR_NOMEM",
           -5: "DGV_ERR_HIP", -6: "DGV_ERR_STATE", -7: "DGV_ERR_IO", -8: "DGV_ERR_UNSUPPORTED"}
 DGV_FP64, DGV_FP32_COMP, DGV_FP32_F64ACC = 0, 1, 2
+PRECISION_NAMES = {DGV_FP64: "FP64", DGV_FP32_COMP: "FP32_COMP", DGV_FP32_F64ACC: "FP32_F64ACC"}
 DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK, DGV_KERNEL_RECORD, DGV_KERNEL_MFMA = 0, 1, 2, 3
 
 

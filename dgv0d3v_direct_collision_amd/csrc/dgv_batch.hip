@@ -78,6 +78,8 @@ struct BatchProjGrid {
 
 struct dgv_batch {
   std::vector<dgv_ctx*> grids;
+  std::vector<uint64_t> a_gen;     // each grid's A-array generation at creation
+  int device = 0;                  // kept here: destroy must not read a grid context (it may be gone)
   int precision = DGV_FP64;
   int32_t ninst = 0;
   std::vector<int32_t> inst_grid, inst_local;
@@ -118,6 +120,23 @@ static int berr(dgv_batch* b, int code, const std::string& m) {
   if (b) b->err = m;
   set_global_error(m);
   return code;
+}
+
+// The batch keeps device pointers into its grids' A-arrays and copies made
+// from them; once a grid's A-array is replaced, truncated or changed in place
+// they are stale, and nothing may run on them.
+bool batch_stale(const dgv_batch* b) {
+  for (size_t g = 0; g < b->grids.size(); ++g)
+    if (!b->grids[g]->has_a || b->grids[g]->a_gen != b->a_gen[g]) return true;
+  return false;
+}
+
+static int batch_check_grids(dgv_batch* b) {
+  for (size_t g = 0; g < b->grids.size(); ++g)
+    if (!b->grids[g]->has_a || b->grids[g]->a_gen != b->a_gen[g])
+      return berr(b, DGV_ERR_STATE, "the A-array of grid " + std::to_string(g) +
+                                        " changed after the batch was created; create a new batch");
+  return DGV_OK;
 }
 #define BHIP(b, call)                                                                          \
   do {                                                                                         \
@@ -669,6 +688,7 @@ static void destroy_plan(dgv_batch* b, BatchPlan& P) {
 // about `tile` pairs; items = segments x chunks of 64 instances.  With `cells`
 // given, only those cells (of every grid) are planned.
 static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std::vector<int32_t>* cells = nullptr) {
+  if (int rc = batch_check_grids(b)) return rc;
   destroy_plan(b, P);
   if (tile <= 0) tile = 1 << 20;
   P.tile = tile;
@@ -766,6 +786,7 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
 
 static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, double* d_q, double scale,
                           bool reduce = true) {
+  if (int rc = batch_check_grids(b)) return rc;
   const int D = b->grids[0]->g.D;
   const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
@@ -883,6 +904,8 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
   }
   dgv_batch* b = new dgv_batch();
   b->grids.assign(grids, grids + ngrids);
+  for (int g = 0; g < ngrids; ++g) b->a_gen.push_back(grids[g]->a_gen);
+  b->device = grids[0]->device;
   b->precision = precision;
   b->ninst = ninstances;
   b->nin.assign(ngrids, 0);
@@ -1022,7 +1045,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
 
 void dgv_batch_destroy(dgv_batch* b) {
   if (!b) return;
-  if (!b->grids.empty()) hipSetDevice(b->grids[0]->device);
+  hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   destroy_plan(b, b->plan);
   destroy_plan(b, b->probe);
@@ -1177,6 +1200,7 @@ int dgv_batch_eval_q_probe(dgv_batch* b) {
 int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_t nsteps, int use_graph) {
   if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
   if (rkmts < 1 || rkmts > 3) return berr(b, DGV_ERR_ARG, "batched stepping supports rkmts 1..3");
+  if (int rc = batch_check_grids(b)) return rc;   // a graph replay would not pass batch_eval_dev
   BHIP(b, hipSetDevice(b->grids[0]->device));
   if (!use_graph) {
     for (int64_t s = 0; s < nsteps; ++s) {

@@ -1,5 +1,6 @@
 // dgv_host.cpp -- host side of the MI355X direct DGV collision library:
 // geometry, A-array layout conversion, work planning, file I/O.
+#include <atomic>
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -132,7 +133,13 @@ int build_grid(const dgv_grid_desc& d, Grid& g, std::string& err) {
   return DGV_OK;
 }
 
+void bump_aarray_generation(dgv_ctx* ctx) {
+  static std::atomic<uint64_t> counter{0};
+  ctx->a_gen = ++counter;
+}
+
 void free_pairblock(dgv_ctx* ctx) {
+  bump_aarray_generation(ctx);
   free_linear(ctx);
   PairBlock& pb = ctx->pb;
   hipFree(pb.d_pairs); hipFree(pb.d_blocks); hipFree(pb.d_group_blocks); hipFree(pb.d_groups);

@@ -180,8 +180,11 @@ struct dgv_ctx {
   double* d_u = nullptr;
   double* d_v = nullptr;
   double* d_w = nullptr;
-  // A-array
+  // A-array.  a_gen changes with every change of the A-array (new, read,
+  // truncated, |A|): batches and surrogates built on the context keep the
+  // value they were built with and refuse to run once it differs.
   bool has_a = false;
+  uint64_t a_gen = 0;
   dgv::PairBlock pb;
   dgv::Plan plan;
   int32_t tile = 0;          // pairs per work item (0 = default)
@@ -215,6 +218,8 @@ struct dgv_ctx {
 namespace dgv {
 
 int set_error(dgv_ctx* ctx, int code, const std::string& msg);
+// gives the context's A-array a new generation number (unique in the process)
+void bump_aarray_generation(dgv_ctx* ctx);
 void set_global_error(const std::string& msg);
 
 #define DGV_HIP(ctx, call)                                                        \
@@ -263,6 +268,9 @@ int prepare_fma_dev(dgv_ctx* ctx, const double* d_fm);
 int eval_linear_dev(dgv_ctx* ctx, const double* d_f, int64_t ldf, double* d_fc, int64_t ldq, int nvec);
 int check_solution_mode(dgv_ctx* ctx, const double* d_f);
 int spatial_oper_device(dgv_ctx* ctx, const double* d_f, double* d_out, double scale);
+
+// true when a grid's A-array changed after the batch was created (dgv_batch.hip)
+bool batch_stale(const dgv_batch* b);
 
 // generator (dgv_agen.hip)
 int generate_pairblock(dgv_ctx* ctx, const dgv_agen_params& p);

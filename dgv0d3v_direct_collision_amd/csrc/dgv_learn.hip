@@ -497,6 +497,7 @@ Gemm plan_gemm(int M, int N, int K, bool allow_split) {
 
 struct dgv_surrogate {
   dgv_ctx* ctx = nullptr;
+  int device = 0;
   int L = 0, batch = 0, Bp = 0;
   std::vector<SurLayer> layers;
   std::vector<uint16_t*> H, Ht;  // H[l]: input of layer l, BF16 [Bp][inp_l] and [inp_l][Bp]
@@ -626,6 +627,7 @@ int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   dgv_surrogate* s = new dgv_surrogate();
   s->ctx = ctx;
+  s->device = ctx->device;
   s->L = nlayers;
   s->batch = batch;
   s->Bp = round_up(batch, 128);
@@ -695,8 +697,10 @@ int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t
 
 void dgv_sur_destroy(dgv_surrogate* s) {
   if (!s) return;
-  hipSetDevice(s->ctx->device);
-  hipStreamSynchronize(s->ctx->stream);
+  // the context may already be destroyed (a caller's finaliser order): use the
+  // device recorded at creation and wait for the whole device, not the context's stream
+  hipSetDevice(s->device);
+  hipDeviceSynchronize();
   free_surrogate(s);
   delete s;
 }
@@ -843,6 +847,10 @@ int dgv_sur_set_probe_cells(dgv_surrogate* s, const int32_t* cells, int32_t ncel
   cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
   for (int32_t c : cl)
     if (c < 0 || c >= g.ncells) return serr(s, DGV_ERR_INDEX, "probe cell outside the grid");
+  if (s->judge && batch_stale(s->judge)) {   // the A-array changed: judge with the current one
+    dgv_batch_destroy(s->judge);
+    s->judge = nullptr;
+  }
   if (!s->judge) {
     std::vector<int32_t> goi(s->batch, 0);
     dgv_ctx* grids[1] = {s->ctx};
@@ -871,6 +879,11 @@ int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows,
   if (!d_f || !d_q || nrows < 0 || ld < s->ctx->g.N || !(tol >= 0)) return serr(s, DGV_ERR_ARG, "eval: bad argument");
   if (!s->judge || s->nprobe_nodes == 0) return serr(s, DGV_ERR_STATE, "no probe cells set");
   SHIP(s, hipSetDevice(s->ctx->device));
+  if (batch_stale(s->judge)) {   // the A-array changed since the probe plan: rebuild it on the current one
+    const std::vector<int32_t> cl = s->probe_cells;
+    int rc = dgv_sur_set_probe_cells(s, cl.data(), (int32_t)cl.size());
+    if (rc) return rc;
+  }
   dgv_ctx* ctx = s->ctx;
   double *jf = nullptr, *jq = nullptr;
   int64_t off = 0, stride = 0;

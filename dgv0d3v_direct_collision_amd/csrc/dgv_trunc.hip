@@ -81,6 +81,7 @@ static int truncate_aarray(dgv_ctx* ctx, const TruncArgs& ta) {
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array");
   PairBlock& pb = ctx->pb;
   const int D = ctx->g.D;
+  bump_aarray_generation(ctx);
   free_linear(ctx);  // fmA and its tables belong to the old A-array
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -192,6 +193,7 @@ int dgv_ctx_aarray_abs(dgv_ctx* ctx) {
   if (!ctx) { set_global_error("null context"); return DGV_ERR_ARG; }
   if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array");
   DGV_HIP(ctx, hipSetDevice(ctx->device));
+  bump_aarray_generation(ctx);
   const int64_t tot = ctx->pb.npairs * (ctx->g.D + 1);
   if (tot > 0)
     hipLaunchKernelGGL(k_abs_coeff, dim3(stride_blocks(tot, 256)), dim3(256), 0, ctx->stream, ctx->pb.npairs,

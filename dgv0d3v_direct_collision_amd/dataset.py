@@ -85,7 +85,7 @@ class TrainingSetWriter:
 
     def run(self, log=None, log_every: int = 1) -> dict:
         """Writes the set; `log(msg)` gets a progress line every `log_every` chunks."""
-        from . import DGV_FP64, DGVBatch
+        from . import PRECISION_NAMES, DGVBatch
         from numpy.lib.format import open_memmap
         g, N, B = self.grid, self.grid.nnodes, self.batch
         t0 = time.perf_counter()
@@ -135,7 +135,7 @@ class TrainingSetWriter:
                 "grid": {"M": [d.Mu, d.Mv, d.Mw], "s": [d.su, d.sv, d.sw],
                          "box": [d.u_L, d.u_R, d.v_L, d.v_R, d.w_L, d.w_R]},
                 "aarray_fingerprint": aarray_fingerprint(g), "workload": g.workload(),
-                "precision": "FP64" if self.precision == DGV_FP64 else "FP32_COMP",
+                "precision": PRECISION_NAMES[self.precision],
                 "seed_base": self.seed_base, "vdf_family": "vdf_from_seed", "batch": B,
                 "seconds": secs, "gpu_seconds": t_gpu}
         with open(os.path.join(self.path, "meta.json"), "w") as fh:

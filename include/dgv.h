@@ -254,6 +254,10 @@ typedef struct dgv_batch dgv_batch;
  * DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in FP64) */
 int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_of_instance,
                      int32_t ninstances, int32_t precision, dgv_batch** out);
+/* A batch reads its grids' A-arrays: once one of them is replaced, truncated or
+ * changed in place, every call that would evaluate returns DGV_ERR_STATE (create
+ * a new batch).  Destroy may come before or after the grids' contexts are
+ * destroyed; no other call may follow a grid context's destruction. */
 void dgv_batch_destroy(dgv_batch* b);
 /* offset (in doubles) of instance i's first value inside the packed buffers and
  * the stride between its consecutive nodes */
@@ -305,6 +309,7 @@ int dgv_batch_eval_q_probe(dgv_batch* b);
 typedef struct dgv_surrogate dgv_surrogate;
 int dgv_sur_create(dgv_ctx* ctx, const int32_t* widths, int32_t nlayers, int32_t batch, uint64_t seed,
                    dgv_surrogate** out);
+/* may come before or after dgv_ctx_destroy of the surrogate's context */
 void dgv_sur_destroy(dgv_surrogate* s);
 int dgv_sur_set_scales(dgv_surrogate* s, double f_scale, double q_scale);
 int dgv_sur_layer_shape(const dgv_surrogate* s, int32_t layer, int32_t* in, int32_t* out);
