@@ -30,41 +30,43 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
+INT8_MFMA_PEAK_TOPS = 5000.0   # MI355X dense int8 matrix
+
+# per batched precision mode: rocprof kernel name, ops per in-grid record visit
+# and VDF, the matrix-core peak and its unit, and the algorithmic coefficient
+# bytes per launch as a function of (records, pairs)
+MODE_ROOFLINE = {
+    "INT8_SPLIT": ("k_batch_i8<27>", 28.0, INT8_MFMA_PEAK_TOPS, "TOP/s", lambda R, P: 260.0 * P),
+    "FP32_COMP": ("k_batch_mfma<27,1>", 2.0, FP32_MFMA_PEAK_TFLOPS, "TFLOP/s", lambda R, P: 12.0 * R),
+    "FP32_F64ACC": ("k_batch_mfma<27,2>", 2.0, FP32_MFMA_PEAK_TFLOPS, "TFLOP/s", lambda R, P: 12.0 * R),
+    "FP64": ("k_batch_mfma<27,0>", 2.0, FP64_PEAK_TFLOPS, "TFLOP/s", lambda R, P: 16.0 * R),
+}
+MODE_OPS_DEF = {
+    "INT8_SPLIT": "2 x 14 int8 digit multiply-adds per in-grid record visit and VDF (five-digit coefficient x "
+                  "four-digit product, digit products down to 256^-4 of the leading one)",
+    "FP32_COMP": "2 flop per in-grid record visit and VDF", "FP32_F64ACC": "2 flop per in-grid record visit and VDF",
+    "FP64": "2 flop per in-grid record visit and VDF"}
+MODE_DTYPE = {"INT8_SPLIT": "i8 digits, i32 exact sums, f64 scaling", "FP32_COMP": "f32+twosum/f64",
+              "FP32_F64ACC": "f32/f64 sums", "FP64": "f64"}
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def maxwellian(u, v, w, n, ub, T):
-    return n * np.exp(-((u - ub[0]) ** 2 + (v - ub[1]) ** 2 + (w - ub[2]) ** 2) / T) / (np.pi * T) ** 1.5
-
-
 def config_vdf(u, v, w, gwts, M, rng):
-    """Seeded bimodal Maxwellian (densities U[0.3,0.7] summing to 1, bulk
-    velocities U[-1,1]^3, temperatures U[0.3,1.0]) + 5% non-negative perturbation
-    on a random 10% of the cells, renormalised to sum(w f) = 1."""
-    n1 = rng.uniform(0.3, 0.7)
-    ub1, ub2 = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
-    T1, T2 = rng.uniform(0.3, 1.0, 2)
-    f = maxwellian(u, v, w, n1, ub1, T1) + maxwellian(u, v, w, 1 - n1, ub2, T2)
-    ncell = M ** 3
-    D = len(f) // ncell
-    cells = rng.choice(ncell, size=max(1, ncell // 10), replace=False)
-    pert = np.zeros(len(f))
-    fmax = f.max()
-    for c in cells:
-        pert[c * D:(c + 1) * D] = 0.05 * rng.uniform(0, 1, D) * fmax
-    f = f + pert
-    return f / np.sum(gwts * f)
+    from dgv0d3v_direct_collision_amd.workloads import perturbed_bimodal_vdf
+    return perturbed_bimodal_vdf(u, v, w, gwts, M, rng)
 
 
 def mixture_vdf(u, v, w, gwts, rng, kmax=3):
-    """Seeded mixture of 1..kmax Maxwellians, unit density."""
-    f = np.zeros(len(u))
-    for _ in range(rng.integers(1, kmax + 1)):
-        f += maxwellian(u, v, w, rng.uniform(0.2, 1.0), rng.uniform(-1, 1, 3), rng.uniform(0.3, 1.0))
-    return f / np.sum(gwts * f)
+    from dgv0d3v_direct_collision_amd.workloads import mixture_vdf as mix
+    return mix(u, v, w, gwts, rng, kmax)
+
+
+def maxwellian(u, v, w, n, ub, T):
+    from dgv0d3v_direct_collision_amd.workloads import maxwellian as mx
+    return mx(u, v, w, n, ub, T)
 
 
 def rel_l2(a, b):
@@ -82,7 +84,9 @@ def main():
     ap.add_argument("--box", type=float, default=4.0)
     ap.add_argument("--nquad", type=int, default=24)
     ap.add_argument("--rk-steps", type=int, default=500)
-    ap.add_argument("--cpu-nodes", type=int, default=1536, help="nodes in the CPU baseline's sample")
+    ap.add_argument("--cpu-cells", type=int, default=48, help="cells in the CPU baseline's 27^3 sample (x 27 nodes)")
+    ap.add_argument("--cpu-nodes-1core", type=int, default=27, help="nodes of the sample timed on one core")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline (the box's share)")
     ap.add_argument("--tile", type=int, default=0, help="batched segment size (0: autotune)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
@@ -93,6 +97,7 @@ def main():
     ap.add_argument("--sur-rows", type=int, default=2048, help="(f, Q) pairs generated for the surrogate")
     ap.add_argument("--sur-steps", type=int, default=400)
     ap.add_argument("--profile", action="store_true", help="evaluator loops only (for rocprofv3)")
+    ap.add_argument("--headline-mode", default="INT8_SPLIT", help="batched mode timed by --profile")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a plain device copy of 1 GiB, 5 times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -153,7 +158,10 @@ def main():
     _, ms_single_main = d.time_eval_q_dev(d_f, d_q, 5)
     log(f"single FP64 ({best}, {tile} pairs/item): {ms_single:.3f} ms/Q wall, main kernel {ms_single_main:.3f} ms")
 
-    # ---- batched: B VDFs, FP32-compensated (headline) and FP64 ------------------------
+    # ---- batched: B VDFs in every precision mode -------------------------------------------
+    # The headline is the fastest mode whose Q of EVERY VDF is within 1e-6 relative
+    # L2 of the FP64 Q (the batched FP64 path, itself held to the single-VDF
+    # kernel and through it to the oracle).
     B = args.batch
     fs = [config_vdf(u, v, w, gw, M, rng) if i % 2 == 0 else mixture_vdf(u, v, w, gw, rng) for i in range(B)]
     fs[0] = f
@@ -165,16 +173,15 @@ def main():
     Q_rows = np.zeros_like(F_rows)
     results = {}
     qb = {}
-    # FP32_COMP is the headline (two-sum compensated FP32 sums); FP32_F64ACC
-    # accumulates the same FP32 matrix-core sums in FP64
-    for prec in (["FP32_COMP"] if args.no_fp64_batch or args.profile else ["FP32_COMP", "FP32_F64ACC", "FP64"]):
+    modes = [args.headline_mode] if args.profile else ["INT8_SPLIT", "FP32_COMP", "FP32_F64ACC", "FP64"]
+    tb = args.tile if args.tile > 0 else 0
+    for prec in modes:
         b = dgv.DGVBatch([d], which, getattr(dgv, "DGV_" + prec))
         b.set_f_rows(d_rows, 0, B)
-        if prec == "FP32_COMP" and args.tile <= 0:
+        if tb <= 0:
             tb, tms = b.autotune()
             log(f"batch {prec}: autotune {tb} pairs/segment ({tms:.1f} ms)")
-        else:   # the given size, or the FP32 path's
-            tb = args.tile if prec == "FP32_COMP" else tb
+        else:
             b.set_tile(tb)
         for _ in range(args.warmup):
             b.eval_q()
@@ -190,57 +197,48 @@ def main():
             b.get_q_rows(d_qrows, 0, B)
             d.dtoh(Q_rows, d_qrows)
             qb[prec] = [Q_rows[i].copy() for i in range(B)]
-            if prec == "FP32_COMP":
-                b.project_invariants()
-                b.get_q_rows(d_qrows, 0, B)
-                d.dtoh(Q_rows, d_qrows)
-                qp = [Q_rows[i].copy() for i in range(B)]
-                psi = np.stack([np.ones(N), u, v, w, u * u + v * v + w * w])
-                worst = 0.0
-                for i in range(B):
-                    mom = (qp[i] * gw) @ psi.T
-                    worst = max(worst, float(np.max(np.abs(mom) / (np.abs(qp[i] * gw) @ np.abs(psi.T)))))
-                results[prec]["projected_moment_rel_max"] = worst
-                log(f"batch {prec}: after FP64 projection max |sum w psi Q| / sum |w psi Q| = {worst:.2e}")
+            b.project_invariants()
+            b.get_q_rows(d_qrows, 0, B)
+            d.dtoh(Q_rows, d_qrows)
+            psi = np.stack([np.ones(N), u, v, w, u * u + v * v + w * w])
+            worst = 0.0
+            for i in range(B):
+                mom = (Q_rows[i] * gw) @ psi.T
+                worst = max(worst, float(np.max(np.abs(mom) / (np.abs(Q_rows[i] * gw) @ np.abs(psi.T)))))
+            results[prec]["projected_moment_rel_max"] = worst
         b.close()
     if args.profile:
         log("profile run done")
         return
 
-    # errors: FP32-comp against FP64 (batched FP64, itself checked against the single-VDF path)
-    err32 = None
-    if "FP64" in qb:
-        e = [rel_l2(qb["FP32_COMP"][i], qb["FP64"][i]) for i in range(B)]
-        emax = [float(np.max(np.abs(qb["FP32_COMP"][i] - qb["FP64"][i])) / np.max(np.abs(qb["FP64"][i])))
-                for i in range(B)]
-        err32 = {"rel_l2_max": max(e), "rel_l2_mean": float(np.mean(e)), "rel_l2_median": float(np.median(e)),
-                 "rel_max_norm_max": max(emax), "frac_vdfs_within_1e-6": float(np.mean(np.array(e) <= 1e-6))}
-        # conditioning of the sum: Qabs = 2/w sum |A| f f (same evaluator on |A|)
-        dabs = dgv.DGV(M, 3, (-L, L))
-        dabs.generate_aarray(min_sens=0.0, nquad=args.nquad)
-        dabs.aarray_abs()
-        kap, sc, sc64 = [], [], []
-        for i in range(B):
-            qa = dabs.EvalCollisionPeriodicAPlus_DGV(fs[i])
-            kap.append(float(np.linalg.norm(qa) / np.linalg.norm(qb["FP64"][i])))
-            sc.append(float(np.linalg.norm(qb["FP32_COMP"][i] - qb["FP64"][i]) / np.linalg.norm(qa)))
-            sc64.append(float(np.linalg.norm(qb["FP32_F64ACC"][i] - qb["FP64"][i]) / np.linalg.norm(qa)))
-        dabs.close()
-        e64 = [rel_l2(qb["FP32_F64ACC"][i], qb["FP64"][i]) for i in range(B)]
-        results["FP32_F64ACC"]["error_vs_fp64"] = {
-            "rel_l2_max": max(e64), "rel_l2_median": float(np.median(e64)),
-            "frac_vdfs_within_1e-6": float(np.mean(np.array(e64) <= 1e-6)),
-            "err_over_term_scale_in_2^-24": max(sc64) * 2.0 ** 24}
-        log(f"FP32 + FP64 accumulation vs FP64: rel L2 max {max(e64):.3e}, error / term scale "
-            f"{max(sc64) * 2 ** 24:.2f} x 2^-24")
-        err32.update({"kappa_median": float(np.median(kap)), "kappa_max": max(kap),
-                      "err_over_term_scale_max": max(sc), "err_over_term_scale_in_2^-24": max(sc) * 2.0 ** 24})
-        log(f"conditioning ||sum|A f f| || / ||Q||: median {np.median(kap):.1f}, max {max(kap):.1f}; "
-            f"FP32-comp error / term scale max {max(sc):.2e} ({max(sc) * 2 ** 24:.2f} x 2^-24)")
-        results["FP64"]["rel_l2_vs_single_vdf"] = rel_l2(qb["FP64"][0], q_single)
-        log(f"FP32-comp vs FP64: rel L2 max {max(e):.3e}; FP64 batch vs single {results['FP64']['rel_l2_vs_single_vdf']:.2e}")
-    else:
-        err32 = {"rel_l2_vs_single_vdf0": rel_l2(qb["FP32_COMP"][0], q_single)}
+    # errors of every mode against FP64, per VDF, with each VDF's conditioning
+    dabs = dgv.DGV(M, 3, (-L, L))
+    dabs.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    dabs.aarray_abs()
+    qabs = [np.linalg.norm(dabs.EvalCollisionPeriodicAPlus_DGV(fs[i])) for i in range(B)]
+    dabs.close()
+    kap = [float(qabs[i] / np.linalg.norm(qb["FP64"][i])) for i in range(B)]
+    for prec in modes:
+        if prec == "FP64":
+            continue
+        e = np.array([rel_l2(qb[prec][i], qb["FP64"][i]) for i in range(B)])
+        emax = [float(np.max(np.abs(qb[prec][i] - qb["FP64"][i])) / np.max(np.abs(qb["FP64"][i]))) for i in range(B)]
+        sc = [float(np.linalg.norm(qb[prec][i] - qb["FP64"][i]) / qabs[i]) for i in range(B)]
+        results[prec]["error_vs_fp64"] = {
+            "rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)), "rel_l2_mean": float(e.mean()),
+            "rel_max_norm_max": max(emax), "frac_vdfs_within_1e-6": float(np.mean(e <= 1e-6)),
+            "err_over_term_scale_max": max(sc), "err_over_term_scale_in_2^-24": max(sc) * 2.0 ** 24}
+        log(f"{prec} vs FP64: rel L2 max {e.max():.3e}, median {np.median(e):.3e}, "
+            f"{np.mean(e <= 1e-6):.3f} of VDFs within 1e-6; error / term scale {max(sc) * 2 ** 24:.4f} x 2^-24")
+    results["FP64"]["rel_l2_vs_single_vdf"] = rel_l2(qb["FP64"][0], q_single)
+    results["FP64"]["error_vs_fp64"] = {"rel_l2_max": 0.0, "frac_vdfs_within_1e-6": 1.0}
+    conditioning = {"kappa_median": float(np.median(kap)), "kappa_max": max(kap),
+                    "definition": "||2/w sum |A| f f|| / ||Q|| per VDF"}
+    log(f"conditioning: kappa median {np.median(kap):.1f}, max {max(kap):.1f}; FP64 batch vs single "
+        f"{results['FP64']['rel_l2_vs_single_vdf']:.2e}")
+    ok = [p for p in modes if results[p]["error_vs_fp64"]["frac_vdfs_within_1e-6"] == 1.0]
+    head = max(ok, key=lambda p: results[p]["vdf_q_per_s"])
+    log(f"headline: {head} (fastest mode with every VDF within 1e-6 of FP64)")
 
     # ---- conservation of the single-VDF Q, RK3 relaxation ---------------------------------
     psi = [np.ones(N), u, v, w, u * u + v * v + w * w]
@@ -274,40 +272,20 @@ def main():
     # ---- CPU baseline ----------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        import oracle
-        l = 13 % d.D
-        log("cpu baseline: exporting one canonical basis function's records")
-        A, xi, xi1 = d.get_aarray_basis(l)
-        G = oracle.Geometry(M, 3, (-L, L))
-        capphi = np.zeros(N, np.int32)
-        capphi[(G.ccell - 1) * d.D + l] = len(A)
-        cells = np.linspace(0, M ** 3 - 1, args.cpu_nodes).astype(np.int64)
-        nodes = cells * d.D + l
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline: oracle on {len(nodes)} nodes x {len(A)} records, {threads} threads")
-        tc = time.perf_counter()
-        qs = oracle.eval_aplus_list(G, capphi, A, xi, xi1, f, nodes, nthreads=threads)
-        cs = time.perf_counter() - tc
-        qps = len(nodes) / (cs * N)
-        err = rel_l2(q_single[nodes], qs)
-        cpu = {"value": qps, "unit": "VDF Q(f) evals/s", "cores": threads, "kind": "port",
-               "sample": f"oracle restatement of EvalCollisionPeriodicAPlus_DGV (FP64, reference loop order) on "
-                         f"{len(nodes)} of {N} nodes (canonical basis {l}, strided cells) of the 27^3 VDF, "
-                         f"{cs:.1f} s on {threads} threads, scaled by N; GPU FP64 vs oracle on the sample: "
-                         f"rel L2 {err:.2e}"}
-        log(f"cpu baseline: {qps:.3e} Q/s ({cs:.1f} s); GPU vs oracle rel L2 {err:.2e}")
+        cpu = cpu_baseline_leg(d, f, q_single, M, L, args)
 
-    r32 = results["FP32_COMP"]
-    value = r32["vdf_q_per_s"]
-    flops = 2.0 * V * B   # one multiply-add per in-grid record visit, per VDF
-    achieved = flops / (r32["kernel_ms"] * 1e-3) / 1e12
+    rh = results[head]
+    value = rh["vdf_q_per_s"]
+    kname, ops_per_visit, peak, punit, coef_bytes = MODE_ROOFLINE[head]
+    ops = ops_per_visit * V * B   # per launch: multiply-adds per in-grid record visit and VDF, x2
+    achieved = ops / (rh["kernel_ms"] * 1e-3) / 1e12
     single_qps = 1e3 / ms_single
-    # the HBM roofline: algorithmic bytes per Q (A records streamed once per
-    # launch: 16 B each in FP64 and 12 B in FP32 with the codes, f read and Q
-    # written once) against 8 TB/s, and the counter-measured traffic
-    bytes_b = 12.0 * R + 16.0 * N * B
+    # the HBM roofline: algorithmic bytes per launch (coefficient records streamed
+    # once per launch, f read and Q written once) against 8 TB/s, and the
+    # counter-measured traffic
+    bytes_b = coef_bytes(R, wl["pairs"]) + 16.0 * N * B
     bytes_1 = 16.0 * R + 16.0 * N
-    tr_b = measured_traffic("k_batch_mfma<27,1>", f"batch{B}_M{M}")
+    tr_b = measured_traffic(kname, f"batch{B}_M{M}")
     tr_1 = measured_traffic("k_pb_mfma<27,false>", f"single_M{M}")
 
     def hbm(bytes_alg, traffic, kernel_ms, rate, per_unit):
@@ -318,35 +296,41 @@ def main():
         if traffic:
             o.update({"measured_gbs": traffic / t / 1e9, "measured_frac": traffic / t / 1e9 / HBM_PEAK_GBS})
         return o
-    hbm_b = hbm(bytes_b, tr_b, r32["kernel_ms"], value, B)
+    hbm_b = hbm(bytes_b, tr_b, rh["kernel_ms"], value, B)
     hbm_1 = hbm(bytes_1, tr_1, ms_single_main, single_qps, 1)
     if rk and rk["projected"].get("steps_per_sec"):
         hbm_1["rk3_steps_per_s_bound_at_hbm_peak"] = hbm_1["rate_bound_at_hbm_peak"] / 3.0
         hbm_1["rk3_frac_of_hbm_roofline"] = rk["projected"]["steps_per_sec"] / hbm_1["rk3_steps_per_s_bound_at_hbm_peak"]
+    for prec in modes:   # every mode against its own matrix-core peak
+        kn, opv, pk, pu, _ = MODE_ROOFLINE[prec]
+        a = opv * V * B / (results[prec]["kernel_ms"] * 1e-3) / 1e12
+        results[prec]["roofline"] = {"kernel": kn, "achieved": a, "peak": pk, "unit": pu, "frac": a / pk,
+                                     "speedup_vs_fp64_plain": results[prec]["vdf_q_per_s"] / single_qps}
     out = {
         "metric": "Q(f) evals/sec at 27^3 nodes: FP64 plain vs batched FP32-comp, rel err <=1e-6",
         "value": value, "unit": "VDF Q(f) evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": r32["ms_per_batch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+twosum/f64", "data": "synthetic",
+        "ms_per_step": rh["ms_per_batch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": MODE_DTYPE[head], "data": "synthetic",
         "config": {
-            "workload": f"batched FP32-compensated Q(f) of {B} VDFs on one 27^3-node grid (3 Gauss x 9 cells per "
-                        f"dim on [-4,4]^3, hard spheres, A-array by the GPU generator), one persistent launch "
-                        f"per step; FP64 final sums",
+            "workload": f"batched Q(f) of {B} VDFs on one 27^3-node grid (3 Gauss x 9 cells per dim on [-4,4]^3, "
+                        f"hard spheres, A-array by the GPU generator), one persistent launch per step, in the "
+                        f"fastest precision mode whose Q of every VDF is within 1e-6 relative L2 of FP64: {head}",
+            "headline_mode": head,
             "M": M, "nodes": N, "batch": B, "records": R, "pairs": wl["pairs"], "visits_per_q": V,
             "agen_seconds": agen_s, "seed": seed,
-            "fp32comp_error_vs_fp64": err32,
+            "conditioning": conditioning,
             "fp64_plain_single_vdf": {"q_per_s": single_qps, "ms_per_q": ms_single, "kernel": best,
                                       "main_kernel_ms": ms_single_main, "pairs_per_item": tile,
                                       "kernels": kernels},
             "batched": results,
-            "speedup_fp32comp_batched_vs_fp64_plain": value / single_qps,
+            "speedup_headline_vs_fp64_plain": value / single_qps,
         },
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": punit,
+                     "frac": achieved / peak,
                      "traffic": tr_b,
-                     "l2": measured_l2("k_batch_mfma<27,1>", f"batch{B}_M{M}"),
-                     "kernel": "k_batch_mfma<27,FP32>", "kernel_ms": r32["kernel_ms"],
-                     "flops_per_launch": flops,
+                     "l2": measured_l2(kname, f"batch{B}_M{M}"),
+                     "kernel": kname, "kernel_ms": rh["kernel_ms"],
+                     "ops_per_launch": ops, "ops_definition": MODE_OPS_DEF[head],
                      "hbm_bytes_per_launch_algorithmic": bytes_b, "hbm": hbm_b,
                      "fp64_plain": {"kernel": knames[best], "kernel_ms": ms_single_main, "hbm": hbm_1,
                                     "tflops": 2.0 * V / (ms_single_main * 1e-3) / 1e12,
@@ -371,6 +355,86 @@ def main():
     d.close()
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(d, f, q_single, M, L, args):
+    """The oracle's restatement of EvalCollisionPeriodicAPlus_DGV (reference loop
+    order and FP64 arithmetic; OpenMP threads over nodes stand in for the
+    reference's MPI ranks over nodes) on the host cores.
+      * 27^3: one Q of the single-VDF configuration's VDF on a node sample of
+        every canonical basis function in evenly spaced cells, scaled to all N
+        nodes (per-node cost: the scan of the basis function's whole record list
+        plus the in-grid visits, so a sample over all basis functions and over
+        boundary and interior cells scales by node count); on the box's core
+        share and on one core.
+      * 15^3 (the plain configuration): one full Q over all 3375 nodes, timed end
+        to end on the core share; the configuration (1 Q + 100 RK2 steps = 201 Q
+        evaluations) is 201 of them, the RK2 combinations being O(N)."""
+    import oracle
+    import dgv0d3v_direct_collision_amd as dgv
+    N, D = d.nnodes, d.D
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, args.cpu_threads)
+    ncell = M ** 3
+    cells = np.unique(np.linspace(0, ncell - 1, args.cpu_cells).astype(np.int64))
+    bases = list(range(D))
+    log(f"cpu baseline: exporting the records of all {D} basis functions (27^3)")
+    counts, A, xi, xi1 = d.get_aarray_bases(bases)
+    G = oracle.Geometry(M, 3, (-L, L))
+    capphi = np.zeros(N, np.int32)
+    for l, n in zip(bases, counts):
+        capphi[(G.ccell - 1) * D + l] = n
+    nodes = np.array([c * D + l for c in cells for l in bases], np.int64)
+    log(f"cpu baseline: oracle on {len(nodes)} of {N} nodes ({len(cells)} cells x {D} basis functions), "
+        f"{threads} threads of {nproc}")
+    tc = time.perf_counter()
+    qs = oracle.eval_aplus_list(G, capphi, A, xi, xi1, f, nodes, nthreads=threads)
+    cs = time.perf_counter() - tc
+    qps = len(nodes) / (cs * N)
+    err = rel_l2(q_single[nodes], qs)
+    n1 = nodes[:: max(1, len(nodes) // args.cpu_nodes_1core)]
+    tc = time.perf_counter()
+    oracle.eval_aplus_list(G, capphi, A, xi, xi1, f, n1, nthreads=1)
+    cs1 = time.perf_counter() - tc
+    qps1 = len(n1) / (cs1 * N)
+    del A, xi, xi1
+    log(f"cpu baseline 27^3: {qps:.3e} Q/s on {threads} threads ({cs:.1f} s), {qps1:.3e} Q/s on 1 core "
+        f"({cs1:.1f} s for {len(n1)} nodes); GPU FP64 vs oracle on the sample: rel L2 {err:.2e}")
+    # the plain configuration at 15^3: a full Q, timed end to end
+    g15 = dgv.DGV(5, 3, (-3.5, 3.5))
+    g15.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    cap15, A15, x15, y15 = g15.get_aarray()
+    G15 = oracle.Geometry(5, 3, (-3.5, 3.5))
+    from dgv0d3v_direct_collision_amd.workloads import bimodal_vdf
+    f15 = bimodal_vdf(g15.nodes_u, g15.nodes_v, g15.nodes_w, np.random.default_rng(20261202))
+    tc = time.perf_counter()
+    q15 = oracle.eval_aplus(G15, cap15, A15, x15, y15, f15, nthreads=threads)
+    c15 = time.perf_counter() - tc
+    err15 = rel_l2(g15.EvalCollisionPeriodicAPlus_DGV(f15), q15)
+    g15.close()
+    log(f"cpu baseline 15^3: one full Q in {c15:.2f} s on {threads} threads; plain configuration (201 Q) "
+        f"{201 * c15:.0f} s; GPU vs oracle rel L2 {err15:.2e}")
+    return {"value": qps, "unit": "VDF Q(f) evals/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": nproc,
+            "value_1core": qps1,
+            "plain_15cubed": {"seconds_per_q": c15, "q_per_s": 1.0 / c15, "config_seconds_201_q": 201 * c15,
+                              "threads": threads, "gpu_vs_oracle_rel_l2": err15},
+            "sample": f"oracle restatement of EvalCollisionPeriodicAPlus_DGV (FP64, reference loop order, OpenMP "
+                      f"threads over nodes) on the 27^3 single-VDF configuration: {len(nodes)} of {N} nodes "
+                      f"({len(cells)} evenly spaced cells x all {D} basis functions), {cs:.1f} s on {threads} of "
+                      f"{nproc} host threads, scaled to N nodes (extrapolated); 1 core: {len(n1)} nodes in "
+                      f"{cs1:.1f} s; GPU FP64 vs oracle on the sample rel L2 {err:.2e}. 15^3 plain configuration: one "
+                      f"full Q in {c15:.2f} s on {threads} threads (config = 201 Q)"}
+
+
 def mixed_grid_leg(dgv, d27, rng, args):
     """512 VDFs of uneven size in one persistent launch: 128 on 15^3 (5 cells,
     [-3.5,3.5]^3), 128 on 21^3 (7 cells, [-4,4]^3), 256 on 27^3 (the main grid);
@@ -388,7 +452,7 @@ def mixed_grid_leg(dgv, d27, rng, args):
     fs = [mixture_vdf(g.nodes_u, g.nodes_v, g.nodes_w, g.nodes_gwts, rng, kmax=3) for g in (grids[k] for k in which)]
     out, qs = {}, {}
     visits = sum(grids[k].workload()["visits"] for k in which)
-    for prec in ("FP32_COMP", "FP32_F64ACC", "FP64"):
+    for prec in ("INT8_SPLIT", "FP32_COMP", "FP32_F64ACC", "FP64"):
         b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
@@ -406,11 +470,11 @@ def mixed_grid_leg(dgv, d27, rng, args):
                      "tflops": 2.0 * visits / el / 1e12, "rk2_graph_step_seconds": rk_el}
         log(f"mixed grids {prec}: {el:.2f} s per 512 VDFs = {len(which) / el:.1f} VDF-Q/s; RK2 step {rk_el:.2f} s")
         b.close()
-    e = np.array([rel_l2(qs["FP32_COMP"][i], qs["FP64"][i]) for i in range(len(which))])
-    out["fp32comp_error_vs_fp64"] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
-                                     "frac_vdfs_within_1e-6": float(np.mean(e <= 1e-6))}
-    e64 = np.array([rel_l2(qs["FP32_F64ACC"][i], qs["FP64"][i]) for i in range(len(which))])
-    out["fp32_f64acc_error_vs_fp64"] = {"rel_l2_max": float(e64.max()), "rel_l2_median": float(np.median(e64))}
+    for prec in ("INT8_SPLIT", "FP32_COMP", "FP32_F64ACC"):
+        e = np.array([rel_l2(qs[prec][i], qs["FP64"][i]) for i in range(len(which))])
+        out[prec]["error_vs_fp64"] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
+                                      "frac_vdfs_within_1e-6": float(np.mean(e <= 1e-6))}
+        log(f"mixed grids {prec} vs FP64: rel L2 max {e.max():.2e}, {np.mean(e <= 1e-6):.3f} within 1e-6")
     out["grids"] = {"15^3": 128, "21^3": 128, "27^3": 256}
     g15.close()
     g21.close()
@@ -439,7 +503,7 @@ def precision_leg(dgv, rng, args):
         amp_of.append(a)
     which = np.zeros(64, np.int32)
     qs, times = {}, {}
-    for prec in ("FP32_COMP", "FP32_F64ACC", "FP64"):
+    for prec in ("INT8_SPLIT", "FP32_COMP", "FP32_F64ACC", "FP64"):
         b = dgv.DGVBatch([g], which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
@@ -449,7 +513,7 @@ def precision_leg(dgv, rng, args):
         b.eval_q()
         times[prec] = time.perf_counter() - t
         qs[prec] = [b.get_q(i) for i in range(64)]
-        if prec == "FP32_COMP":
+        if prec == "INT8_SPLIT":
             b.project_invariants()
             psi = np.stack([np.ones(len(u)), u, v, w, u * u + v * v + w * w])
             qp = [b.get_q(i) for i in range(64)]
@@ -464,13 +528,20 @@ def precision_leg(dgv, rng, args):
         kap = [float(np.linalg.norm(g.EvalCollisionPeriodicAPlus_DGV(fs[i])) / np.linalg.norm(qs["FP64"][i]))
                for i in idx]
         e64 = [rel_l2(qs["FP32_F64ACC"][i], qs["FP64"][i]) for i in idx]
+        ei8 = [rel_l2(qs["INT8_SPLIT"][i], qs["FP64"][i]) for i in idx]
+        emi8 = [float(np.max(np.abs(qs["INT8_SPLIT"][i] - qs["FP64"][i])) / np.max(np.abs(qs["FP64"][i])))
+                for i in idx]
         per[f"{a:g}"] = {"vdfs": len(idx), "rel_l2_max": max(e), "rel_l2_median": float(np.median(e)),
                          "rel_max_norm_max": max(em), "kappa_median": float(np.median(kap)),
-                         "fp32_f64acc_rel_l2_max": max(e64)}
-        log(f"precision 33^3, amplitude {a:g}: FP32-comp rel L2 max {max(e):.2e}, median {np.median(e):.2e}, "
-            f"kappa median {np.median(kap):.0f}")
+                         "kappa_max": max(kap),
+                         "fp32_f64acc_rel_l2_max": max(e64),
+                         "int8_split_rel_l2_max": max(ei8), "int8_split_rel_l2_median": float(np.median(ei8)),
+                         "int8_split_rel_max_norm_max": max(emi8)}
+        log(f"precision 33^3, amplitude {a:g}: INT8_SPLIT rel L2 max {max(ei8):.2e}; FP32-comp rel L2 max "
+            f"{max(e):.2e}, median {np.median(e):.2e}; kappa median {np.median(kap):.0f}")
     g.close()
     return {"nodes": 35937, "vdfs": 64, "per_perturbation_amplitude": per,
+            "int8_split_vdf_q_per_s": 64 / times["INT8_SPLIT"],
             "fp32comp_vdf_q_per_s": 64 / times["FP32_COMP"], "fp64_vdf_q_per_s": 64 / times["FP64"],
             "fp32_f64acc_vdf_q_per_s": 64 / times["FP32_F64ACC"],
             "projected_moment_rel_max": proj}

@@ -16,9 +16,11 @@ LIB_PATH = os.path.join(HERE, "libdgv_amd.so")
 DGV_OK = 0
 ERRORS = {-1: "DGV_ERR_ARG", -2: "DGV_ERR_GRID", -3: "DGV_ERR_INDEX", -4: "DGV_ERR_NOMEM",
           -5: "DGV_ERR_HIP", -6: "DGV_ERR_STATE", -7: "DGV_ERR_IO", -8: "DGV_ERR_UNSUPPORTED"}
-DGV_FP64, DGV_FP32_COMP, DGV_FP32_F64ACC = 0, 1, 2
-PRECISION_NAMES = {DGV_FP64: "FP64", DGV_FP32_COMP: "FP32_COMP", DGV_FP32_F64ACC: "FP32_F64ACC"}
+DGV_FP64, DGV_FP32_COMP, DGV_FP32_F64ACC, DGV_INT8_SPLIT = 0, 1, 2, 3
+PRECISION_NAMES = {DGV_FP64: "FP64", DGV_FP32_COMP: "FP32_COMP", DGV_FP32_F64ACC: "FP32_F64ACC",
+                   DGV_INT8_SPLIT: "INT8_SPLIT"}
 DGV_KERNEL_AUTO, DGV_KERNEL_PAIRBLOCK, DGV_KERNEL_RECORD, DGV_KERNEL_MFMA = 0, 1, 2, 3
+DGV_FRAMING_SUBRECORDS, DGV_FRAMING_UNSIGNED32 = 0, 1
 
 
 class GridDesc(C.Structure):
@@ -65,8 +67,18 @@ SIGNATURES = {
     "dgv_ctx_aarray_size": (C.c_int, [_vp, _lp, _lp]),
     "dgv_ctx_workload": (C.c_int, [_vp, _lp]),
     "dgv_ctx_get_aarray_basis": (C.c_int, [_vp, _i32, _dp, _ip, _ip, _i64, _lp, C.c_int]),
+    "dgv_ctx_get_aarray_bases": (C.c_int, [_vp, _ip, _i32, _dp, _ip, _ip, _i64, _lp, C.c_int]),
     "dgv_aarray_write_file": (C.c_int, [_vp, C.c_char_p]),
     "dgv_ctx_read_aarray_file": (C.c_int, [_vp, C.c_char_p]),
+    "dgv_aarray_file_header": (C.c_int, [C.c_char_p, _lp, _lp]),
+    "dgv_aarray_file_read": (C.c_int, [C.c_char_p, _ip, _i64, _dp, _ip, _ip, _ip, _i64]),
+    "dgv_aarray_file_write": (C.c_int, [C.c_char_p, _ip, _i64, _dp, _ip, _ip, _ip, _i64, _i32]),
+    "dgv_set_fortran_subrecord_limit": (C.c_int, [_i64]),
+    "dgv_aarray_chunks_scan": (C.c_int, [C.POINTER(GridDesc), C.c_char_p, C.c_char_p, _i32, _i32, _i32, _lp]),
+    "dgv_aarray_chunks_read": (C.c_int, [C.POINTER(GridDesc), C.c_char_p, C.c_char_p, _i32, _i32, _i32, _i64,
+                                         _i64, _i32, _ip, _i64, _dp, _ip, _ip, _ip, _i64, _lp]),
+    "dgv_ctx_write_aarray_chunks": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _i32, _i32, _i64, _i32, _ip]),
+    "dgv_ctx_read_aarray_chunks": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _i32, _i32, _i32]),
     "dgv_ctx_trunc_aarray_threshold": (C.c_int, [_vp, _dbl]),
     "dgv_ctx_trunc_aarray_radius": (C.c_int, [_vp, _dbl]),
     "dgv_ctx_aarray_abs": (C.c_int, [_vp]),

@@ -81,6 +81,75 @@ def chunk_name(M, s, box, directory="", base="", chunk=0, k_c=2, N_x=3) -> str:
     return buf.value.decode()
 
 
+def _lptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def aarray_file_header(path: str):
+    """(mm, nn) of an A-array file: record 1 of WriteAarraysDGV's layout."""
+    mm, nn = C.c_int64(), C.c_int64()
+    check(load().dgv_aarray_file_header(path.encode(), C.byref(mm), C.byref(nn)))
+    return mm.value, nn.value
+
+
+def aarray_file_read(path: str):
+    """ReadAarraysDGV on host arrays (readwrite.f90:1066-1118): returns
+    (A_capphi, A, A_xi, A_xi1, A_phi); records above 2^31 - 9 bytes in either
+    framing (Intel / gfortran subrecords, or LLVM Flang's single record)."""
+    mm, nn = aarray_file_header(path)
+    cap = np.zeros(mm, np.int32)
+    A = np.zeros(nn)
+    xi, xi1, phi = (np.zeros(nn, np.int32) for _ in range(3))
+    check(load().dgv_aarray_file_read(path.encode(), iptr(cap), mm, dptr(A), iptr(xi), iptr(xi1), iptr(phi), nn))
+    return cap, A, xi, xi1, phi
+
+
+def aarray_file_write(path: str, capphi, A, A_xi, A_xi1, A_phi, framing: int = _lib.DGV_FRAMING_SUBRECORDS):
+    """WriteAarraysDGV's two records (readwrite.f90:1459-1484) from host arrays."""
+    cap = np.ascontiguousarray(capphi, np.int32)
+    A = np.ascontiguousarray(A, np.float64)
+    xi, xi1, phi = (np.ascontiguousarray(x, np.int32) for x in (A_xi, A_xi1, A_phi))
+    check(load().dgv_aarray_file_write(path.encode(), iptr(cap), len(cap), dptr(A), iptr(xi), iptr(xi1), iptr(phi),
+                                       len(A), framing))
+
+
+def set_fortran_subrecord_limit(nbytes: int = 2147483639):
+    """Largest subrecord payload of the Intel / gfortran framing (a test hook)."""
+    check(load().dgv_set_fortran_subrecord_limit(nbytes))
+
+
+def ScanAarrsChnks4Acapphi(M, s, box, nchnks: int, directory="", base="", k_c=2, N_x=3):
+    """Records per chunk of a chunk set (readwrite.f90:2292-2365)."""
+    g = grid_desc(M, s, box)
+    act = np.zeros(nchnks, np.int64)
+    check(load().dgv_aarray_chunks_scan(C.byref(g), directory.encode(), base.encode(), k_c, N_x, nchnks, _lptr(act)))
+    return act
+
+
+def ReadAarraysChnksOfstNrecDGV(M, s, box, fstchnk: int, ofst: int, numrec: int, nchnks: int, directory="",
+                                base="", k_c=2, N_x=3):
+    """numrec records of a chunk set from chunk fstchnk (1-based) on, after the
+    first ofst records of that chunk (readwrite.f90:1261-1449): returns
+    (A_capphi, A, A_xi, A_xi1, A_phi) as the reference's arrays hold them."""
+    g = grid_desc(M, s, box)
+    mm, _ = aarray_file_header(chunk_name(M, s, box, directory, base, fstchnk - 1, k_c, N_x))
+    cap = np.zeros(mm, np.int32)
+    A = np.zeros(numrec)
+    xi, xi1, phi = (np.zeros(numrec, np.int32) for _ in range(3))
+    nread = C.c_int64()
+    check(load().dgv_aarray_chunks_read(C.byref(g), directory.encode(), base.encode(), k_c, N_x, fstchnk, ofst,
+                                        numrec, nchnks, iptr(cap), mm, dptr(A), iptr(xi), iptr(xi1), iptr(phi),
+                                        numrec, C.byref(nread)))
+    n = nread.value
+    return cap, A[:n], xi[:n], xi1[:n], phi[:n]
+
+
+def ReadAarraysChnksDGV(M, s, box, nchnks: int, directory="", base="", k_c=2, N_x=3):
+    """Every record of a chunk set (readwrite.f90:1130-1243)."""
+    total = int(ScanAarrsChnks4Acapphi(M, s, box, nchnks, directory, base, k_c, N_x).sum())
+    return ReadAarraysChnksOfstNrecDGV(M, s, box, 1, 0, total, nchnks, directory, base, k_c, N_x)
+
+
 def sol_file_name(M, s, box, suff, directory="", base="", k_c=1, N_x=1) -> str:
     """WriteSol_DGV / RestartSolutionDGV file name: MakeBaseNameDGV
     (readwrite.f90:1505-1560) + "_time" + suff + "_sol.dat"."""
@@ -242,6 +311,20 @@ class DGV:
                                                    n.value, C.byref(n), index_base))
         return A, xi, xi1
 
+    def get_aarray_bases(self, ls, index_base: int = 1):
+        """Records of the canonical basis functions ``ls`` (0-based), each list in
+        the reference's serial order, from one pass over the device layout:
+        returns (counts, A, A_xi, A_xi1) with the lists back to back."""
+        ls = np.ascontiguousarray(ls, np.int32)
+        counts = np.zeros(len(ls), np.int64)
+        self._chk(self._L.dgv_ctx_get_aarray_bases(self._ctx, iptr(ls), len(ls), None, None, None, 0,
+                                                   counts.ctypes.data_as(C.POINTER(C.c_int64)), index_base))
+        n = int(counts.sum())
+        A, xi, xi1 = np.zeros(n), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self._chk(self._L.dgv_ctx_get_aarray_bases(self._ctx, iptr(ls), len(ls), dptr(A), iptr(xi), iptr(xi1), n,
+                                                   counts.ctypes.data_as(C.POINTER(C.c_int64)), index_base))
+        return counts, A, xi, xi1
+
     def get_aarray(self, index_base: int = 1):
         """The whole A-array in the reference layout (capphi, A, A_xi, A_xi1)."""
         capphi = np.zeros(self.nnodes, np.int32)
@@ -259,6 +342,19 @@ class DGV:
 
     def read_aarray(self, path: str):
         self._chk(self._L.dgv_ctx_read_aarray_file(self._ctx, path.encode()))
+
+    def write_aarray_chunks(self, directory: str, base: str, max_records: int, k_c: int = 2, N_x: int = 3,
+                            framing: int = _lib.DGV_FRAMING_SUBRECORDS) -> int:
+        """The A-array as a chunk set of at most max_records records per chunk,
+        named by MakeBaseNameAoperChnksDGV; returns the number of chunks."""
+        n = C.c_int32()
+        self._chk(self._L.dgv_ctx_write_aarray_chunks(self._ctx, directory.encode(), base.encode(), k_c, N_x,
+                                                      max_records, framing, C.byref(n)))
+        return n.value
+
+    def read_aarray_chunks(self, nchnks: int, directory: str, base: str, k_c: int = 2, N_x: int = 3):
+        """ReadAarraysChnksDGV(nchnks) (readwrite.f90:1130-1243) into this context."""
+        self._chk(self._L.dgv_ctx_read_aarray_chunks(self._ctx, directory.encode(), base.encode(), k_c, N_x, nchnks))
 
     def TruncAarrsTrhls_DGV(self, trhld: float):
         """Drops the records with |A| <= trhld (dgvtools_mod.f90:1313-1362)."""

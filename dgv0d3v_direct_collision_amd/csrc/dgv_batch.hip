@@ -18,10 +18,14 @@
 // and f are rounded to FP32, products and the per-block sums run on
 // v_mfma_f32_16x16x4_f32, and the sums of every 32 pairs are folded into an
 // FP32 (hi, lo) pair with the two-sum of Knuth; the item writes hi + lo in FP64.
-// FP64: two waves per item, block sums added into an FP64 running total.  A second
+// FP64: two waves per item, block sums added into an FP64 running total.
+// INT8_SPLIT: the same contraction in exact integer arithmetic (k_batch_i8).  A
+// second
 // kernel adds the items of each (cell, instance) in a fixed order in FP64 and
 // applies 2/w (and the spatial operator's scale).
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -37,7 +41,14 @@ struct BatchGridDev {
   const BlockHdr* blocks;
   const int32_t* blist;    // per-cell block lists, concatenated
   const double* gwts;
+  // INT8_SPLIT: coefficient digits, pair codes, per-block row scales and the
+  // first padded pair of each block (blocks padded to multiples of 16 pairs)
+  const uint32_t* digits;
+  const uint32_t* codes;
+  const double* bscale;
+  const int64_t* pbeg16;
   int64_t foff;            // offset of this grid in the packed buffers
+  int64_t eoff;            // offset of this grid in the per-(cell, instance) exponents (foff / D)
   int32_t Bpad, Mv, Mw, D;
 };
 
@@ -94,6 +105,12 @@ struct dgv_batch {
   std::vector<std::vector<int64_t>> cell_ptr;  // per grid: ncells+1 offsets into blist
   std::vector<std::vector<int32_t>> h_blist;   // per grid: host copy of the per-cell block lists
   std::vector<float*> d_pairs32;
+  // INT8_SPLIT: per grid, the coefficient digits (d_i8[g][0]), pair codes [1],
+  // block row scales [2] and padded block starts [3]; per evaluation, f scaled
+  // per (cell, instance) by a power of two (d_fhat) and the exponents (d_fexp)
+  std::vector<std::array<void*, 4>> d_i8;
+  double* d_fhat = nullptr;
+  int32_t* d_fexp = nullptr;
   dgv::BatchGridDev* d_grids = nullptr;
   int32_t waves_per_cu = 0;
   dgv::BatchPlan plan;             // all cells
@@ -523,6 +540,310 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   }
 }
 
+// ---- INT8_SPLIT: exact integer digit products on the int8 matrix cores ----------------
+//
+// Per block and basis row l the coefficients are fixed point with the block-row
+// exponent eA (max |T_l| < 2^eA): At = rint(T 2^(BA - eA)), |At| < 2^38, written
+// as five signed base-256 digits A_0 (most significant) .. A_4.  Per output cell
+// c the f of every (cell, instance) is scaled by a power of two so that its
+// largest value lies in [2^14, 2^15): fh = f 2^(15 - e).  A pair's product
+// x = fh(c+a, p) fh(c+b, q), |x| < 2^30, is rounded to an integer and written as
+// four signed digits X_0 .. X_3 in the bytes of one dword (X_0 in the top byte),
+// by rounding x + 0x80808080 to an unsigned integer and flipping each byte's top
+// bit.  Then At x = sum_k 256^(7-k) sum_{t+s=k} A_t X_s, and the levels k = 0..4
+// (all digit products down to 256^-4 of the leading one) are accumulated exactly
+// in int32 by v_mfma_i32_16x16x64_i8:
+//   level 3: A dword (A_0, A_1, A_2, A_3) against the natural x dword;
+//   level 4: A dword (A_1, A_2, A_3, A_4) against the natural x dword;
+//   level k < 3: the first A dword against the x dword shifted right by 8 (3 - k).
+// At the end of a block the five int32 sums are combined in FP64 and scaled by
+// 2^(eA - 38) 2^(e(c+a) + e(c+b) - 30) into the item's FP64 running sum.
+// Dropped: digit products below 256^-4 of the leading one and the roundings of
+// At and x, i.e. relative errors near 2^-30 per term at the block's scale.
+typedef int bi4 __attribute__((ext_vector_type(4)));
+typedef unsigned int bu4 __attribute__((ext_vector_type(4)));
+constexpr int kI8BA = 38;                   // coefficient bits: |At| < 2^38
+constexpr int kI8BX = 30;                   // product bits: |x| < 2^30
+constexpr int kI8FS = 15;                   // f scaled into [2^14, 2^15)
+constexpr double kI8Bias = 2155905152.5;    // 0x80808080 + 1/2 (rounds to nearest)
+
+template <int D>
+constexpr int i8_dwords_per_pair() { return ((D + 15) / 16) * 16 * 2; }   // 2 dwords per tile row
+
+// One workgroup per block: the block-row exponents, the digit dwords of every
+// (pair, row) of the block padded to 16 pairs, and the pairs' LDS row offsets.
+// Layout of a 16-pair step s of a block: for each row tile rt, 64 lanes x 8
+// dwords, lane 16 g + i holding row i's (A_0..A_3) dwords of pairs 4 g .. 4 g + 3
+// then its (A_1..A_4) dwords, i.e. the A operands of the step's MFMAs.
+template <int D>
+__global__ __launch_bounds__(256) void k_i8_digits(const BlockHdr* __restrict__ blocks,
+                                                   const int64_t* __restrict__ pbeg16,
+                                                   const double* __restrict__ pairs64,
+                                                   uint32_t* __restrict__ digits, uint32_t* __restrict__ codes,
+                                                   double* __restrict__ bscale) {
+  constexpr int RT = (D + 15) / 16;
+  const int blk = blockIdx.x;
+  const int n = blocks[blk].npairs;
+  const int64_t p0 = blocks[blk].pbeg;
+  const int64_t o16 = pbeg16[blk];
+  const int n16 = (n + 15) & ~15;
+  __shared__ int emax[16 * RT];
+  if (threadIdx.x < 16 * RT) emax[threadIdx.x] = INT_MIN;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n * D; i += blockDim.x) {
+    const int j = i / D, l = i - (i / D) * D;
+    const double a = pairs64[(p0 + j) * (D + 1) + l];
+    if (a != 0.0) {
+      int e;
+      frexp(a, &e);
+      atomicMax(&emax[l], e);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 16 * RT) {
+    const int l = threadIdx.x;
+    bscale[(int64_t)blk * 16 * RT + l] = (l < D && emax[l] != INT_MIN) ? ldexp(1.0, emax[l] - kI8BA) : 0.0;
+  }
+  uint32_t* dblk = digits + o16 * i8_dwords_per_pair<D>();
+  for (int i = threadIdx.x; i < n16 * 16 * RT; i += blockDim.x) {
+    const int j = i / (16 * RT), r = i - j * (16 * RT);
+    long long v = 0;
+    if (j < n && r < D && emax[r] != INT_MIN) v = llrint(ldexp(pairs64[(p0 + j) * (D + 1) + r], kI8BA - emax[r]));
+    int dg[5];
+#pragma unroll
+    for (int k = 4; k >= 0; --k) {
+      const int d = (int)((v + 128) & 255) - 128;
+      dg[k] = d;
+      v = (v - d) >> 8;
+    }
+    const uint32_t w1 = (uint32_t)(uint8_t)dg[0] | ((uint32_t)(uint8_t)dg[1] << 8) |
+                        ((uint32_t)(uint8_t)dg[2] << 16) | ((uint32_t)(uint8_t)dg[3] << 24);
+    const uint32_t w2 = (uint32_t)(uint8_t)dg[1] | ((uint32_t)(uint8_t)dg[2] << 8) |
+                        ((uint32_t)(uint8_t)dg[3] << 16) | ((uint32_t)(uint8_t)dg[4] << 24);
+    const int st = j >> 4, k = j & 15, g = k >> 2, m = k & 3, rt = r >> 4, ii = r & 15;
+    uint32_t* o = dblk + ((int64_t)(st * RT + rt) * 64 + 16 * g + ii) * 8;
+    o[m] = w1;
+    o[4 + m] = w2;
+  }
+  for (int j = threadIdx.x; j < n16; j += blockDim.x) {
+    uint32_t c = 0;   // padded pairs: rows 0 and 0, zero digits
+    if (j < n) c = (uint32_t)((uint64_t)__double_as_longlong(pairs64[(p0 + j) * (D + 1) + D]) >> 32);
+    codes[o16 + j] = c;
+  }
+}
+
+// Per (cell, instance) of one grid: e = exponent of max |f| over the cell's D
+// nodes (max |f| < 2^e), fh = f 2^(15 - e) (exact), both for k_batch_i8.
+template <int D>
+__global__ void k_i8_fixed(int64_t ncells, int32_t Bpad, const double* __restrict__ F, double* __restrict__ Fh,
+                           int32_t* __restrict__ E) {
+  const int64_t n = ncells * Bpad, step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
+    const int64_t c = t / Bpad, v = t - c * Bpad;
+    double m = 0.0;
+#pragma unroll
+    for (int p = 0; p < D; ++p) m = fmax(m, fabs(F[(c * D + p) * Bpad + v]));
+    int e = -1000;
+    if (m > 0.0) frexp(m, &e);
+    E[t] = e;
+#pragma unroll
+    for (int p = 0; p < D; ++p) Fh[(c * D + p) * Bpad + v] = ldexp(F[(c * D + p) * Bpad + v], kI8FS - e);
+  }
+}
+
+// The INT8_SPLIT evaluator.  Workgroup of four waves per item, two chunks of 64
+// instances per item, two waves per chunk (32 instances each: column tiles
+// ct = 0, 1, column jj of tile ct = instance 2 jj + ct of the wave's 32), as the
+// FP64 mode.  A 16-pair step: per row tile 5 level MFMAs per column tile.
+template <int D>
+__global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ items, int nitems,
+                                                  const BatchGridDev* __restrict__ grids,
+                                                  const double* __restrict__ fhat, const int32_t* __restrict__ fexp,
+                                                  double* __restrict__ partial, int* __restrict__ counter) {
+  constexpr int RT = (D + 15) / 16;
+  constexpr int CH = 2, WPC = 2, CT = 2;
+  constexpr int DW = i8_dwords_per_pair<D>();     // digit dwords per pair
+  typedef double VB __attribute__((ext_vector_type(2)));
+  __shared__ double lfa[CH][D * 64], lfb[CH][D * 64];
+  __shared__ int s_item;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kk = lane >> 4, jj = lane & 15;
+  const int ch = wave / WPC, sub = wave % WPC;
+  const int col0 = sub * 32;
+  const int lcol = col0 + 2 * jj;                  // the lane's instances in a staged row: lcol, lcol + 1
+  const bi4 izero = bi4{0, 0, 0, 0};
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int id = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (id >= nitems) return;
+    const BatchItem it = items[id];
+    const BatchGridDev G = grids[it.grid];
+    const bool active = ch < it.nvc;
+    const int Mv = G.Mv, Mw = G.Mw;
+    const int c = it.cell;
+    const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
+    const int64_t vb = (int64_t)(it.vc0 + ch) * 64 + lane;          // the instance this thread stages
+    const int64_t vcol = (int64_t)(it.vc0 + ch) * 64 + lcol;        // the lane's first column's instance
+    double tot[RT][CT][4];
+    bi4 acc[5][RT][CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tot[rt][ct][e] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k][rt][ct] = izero;
+      }
+    double* la = lfa[ch];
+    double* lb = lfb[ch];
+    int prev_xc = -1;
+    for (int bi = it.b0; bi < it.b1; ++bi) {
+      const int blk = G.blist[bi];
+      const BlockHdr* B = G.blocks + blk;
+      const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
+      const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
+      const bool new_x = xc != prev_xc;
+      prev_xc = xc;
+      const int n = B->npairs;
+      const int nst = (n + 15) >> 4;
+      const int64_t o16 = G.pbeg16[blk];
+      const bi4* Ablk = reinterpret_cast<const bi4*>(G.digits + o16 * DW) + lane * 2;
+      const bu4* Cblk = reinterpret_cast<const bu4*>(G.codes + o16) + kk;
+      // scales of the block: the lane's output rows (4 kk + e of tile rt) and columns
+      double sA[RT][4], sF[CT] = {};
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sA[rt][e] = G.bscale[(int64_t)blk * 16 * RT + 16 * rt + 4 * kk + e];
+      if (active) {   // an idle chunk's columns lie past the grid's instances
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int ea = fexp[G.eoff + (int64_t)xc * G.Bpad + vcol + ct];
+          const int eb = fexp[G.eoff + (int64_t)yc * G.Bpad + vcol + ct];
+          sF[ct] = ldexp(1.0, ea + eb - 2 * kI8FS);
+        }
+      }
+      // first step's operands ahead of the staging
+      bi4 a1[RT], a2[RT];
+      bu4 cd;
+      if (active && n > 0) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          a1[rt] = Ablk[rt * 128];
+          a2[rt] = Ablk[rt * 128 + 1];
+        }
+        cd = Cblk[0];
+      }
+      __syncthreads();
+      if (active) {   // waves 2c and 2c+1 split the rows of chunk c
+        const double* F = fhat + G.foff;
+        constexpr int NS = (D + WPC - 1) / WPC;
+        double sa[NS], sb[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+          const int p = min(sub + i * WPC, D - 1);
+          if (new_x) sa[i] = F[((int64_t)xc * D + p) * G.Bpad + vb];
+          sb[i] = F[((int64_t)yc * D + p) * G.Bpad + vb];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+          const int p = sub + i * WPC;
+          if (p < D) {
+            if (new_x) la[p * 64 + lane] = sa[i];
+            lb[p * 64 + lane] = sb[i];
+          }
+        }
+      }
+      __syncthreads();
+      if (!active || n == 0) continue;
+      const char* la_lane = reinterpret_cast<const char*>(la + lcol);
+      const char* lb_lane = reinterpret_cast<const char*>(lb + lcol);
+      for (int st = 0; st < nst; ++st) {
+        // the x dwords of the lane's four pairs, for both column tiles
+        bi4 nat[CT];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const uint32_t code = cd[m];
+          const VB ra = *reinterpret_cast<const VB*>(la_lane + (code & 0xffffu));
+          const VB rb = *reinterpret_cast<const VB*>(lb_lane + (code >> 16));
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct) {
+            const double y = fma(ra[ct], rb[ct], kI8Bias);
+            nat[ct][m] = (int)(__double2uint_rz(y) ^ 0x80808080u);
+          }
+        }
+        // the next step's operands
+        bi4 n1[RT], n2[RT];
+        bu4 ncd = cd;
+        if (st + 1 < nst) {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            n1[rt] = Ablk[((st + 1) * RT + rt) * 128];
+            n2[rt] = Ablk[((st + 1) * RT + rt) * 128 + 1];
+          }
+          ncd = Cblk[(st + 1) * 4];
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const bu4 u3 = __builtin_bit_cast(bu4, nat[ct]);
+          const bi4 x2 = __builtin_bit_cast(bi4, u3 >> 8u);
+          const bi4 x1 = __builtin_bit_cast(bi4, u3 >> 16u);
+          const bi4 x0 = __builtin_bit_cast(bi4, u3 >> 24u);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x0, acc[0][rt][ct], 0, 0, 0);
+            acc[1][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x1, acc[1][rt][ct], 0, 0, 0);
+            acc[2][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x2, acc[2][rt][ct], 0, 0, 0);
+            acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], nat[ct], acc[3][rt][ct], 0, 0, 0);
+            acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a2[rt], nat[ct], acc[4][rt][ct], 0, 0, 0);
+          }
+        }
+        if (st + 1 < nst) {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            a1[rt] = n1[rt];
+            a2[rt] = n2[rt];
+          }
+          cd = ncd;
+        }
+      }
+      // the block's exact sums, combined and scaled into the FP64 running sums
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            double v = (double)acc[0][rt][ct][e];
+            v = fma(v, 256.0, (double)acc[1][rt][ct][e]);
+            v = fma(v, 256.0, (double)acc[2][rt][ct][e]);
+            v = fma(v, 256.0, (double)acc[3][rt][ct][e]);
+            v = fma(v, 256.0, (double)acc[4][rt][ct][e]);
+            tot[rt][ct][e] = fma(v, sA[rt][e] * sF[ct], tot[rt][ct][e]);
+          }
+#pragma unroll
+          for (int k = 0; k < 5; ++k) acc[k][rt][ct] = izero;
+        }
+    }
+    if (!active) continue;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int v = lcol + ct;
+        double* out = partial + (it.prow + 64 * ch + v) * D;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int l = 16 * rt + 4 * kk + e;   // int32 16x16 output: row 4 (lane / 16) + e
+          if (l < D) out[l] = tot[rt][ct][e];
+        }
+      }
+  }
+}
+
 // One wave per (grid, cell, chunk): lane v sums its instance's rows over the
 // cell's items in order, Q = (2 s) / w * scale, written to the packed layout.
 template <int D>
@@ -722,7 +1043,8 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
           acc = 0;
         }
       }
-      const int chpi = b->precision != DGV_FP64 ? chunks_per_item<true>() : chunks_per_item<false>();
+      const bool fp32 = b->precision == DGV_FP32_COMP || b->precision == DGV_FP32_F64ACC;
+      const int chpi = fp32 ? chunks_per_item<true>() : chunks_per_item<false>();   // INT8_SPLIT: as FP64
       for (int vc0 = 0; vc0 < nchunk; vc0 += chpi) {
         const int nvc = std::min(chpi, nchunk - vc0);
         std::vector<int64_t> first;   // first partial row of each segment's item
@@ -790,8 +1112,21 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
   const int D = b->grids[0]->g.D;
   const int nitems = (int)P.items.size();
   hipStream_t st = b->stream;
-  if (b->precision != DGV_FP64)
+  if (b->precision == DGV_FP32_COMP || b->precision == DGV_FP32_F64ACC)
     hipLaunchKernelGGL(k_to_f32, dim3(stride_blocks(b->total, 256)), dim3(256), 0, st, b->total, d_f, b->d_f32);
+  if (b->precision == DGV_INT8_SPLIT)
+    for (size_t g = 0; g < b->grids.size(); ++g) {
+      const int64_t nc = b->grids[g]->g.ncells;
+      const int64_t n = nc * b->Bpad[g];
+      if (n == 0) continue;
+      const double* F = d_f + b->foff[g];
+      double* Fh = b->d_fhat + b->foff[g];
+      int32_t* E = b->d_fexp + b->foff[g] / D;
+      if (D == 27)
+        hipLaunchKernelGGL(k_i8_fixed<27>, dim3(stride_blocks(n, 256)), dim3(256), 0, st, nc, b->Bpad[g], F, Fh, E);
+      else
+        hipLaunchKernelGGL(k_i8_fixed<8>, dim3(stride_blocks(n, 256)), dim3(256), 0, st, nc, b->Bpad[g], F, Fh, E);
+    }
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
   // workgroups of four waves; waves_per_cu sets the persistent grid's size
   const int wpc = b->waves_per_cu > 0 ? b->waves_per_cu : 8;
@@ -801,7 +1136,14 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
 #define DGV_BATCH_LAUNCH(DD, MM)                                                                          \
   hipLaunchKernelGGL((k_batch_mfma<DD, MM>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f, \
                      b->d_f32, P.d_partial, b->d_counter)
-    if (D == 27) {
+    if (b->precision == DGV_INT8_SPLIT) {
+      if (D == 27)
+        hipLaunchKernelGGL(k_batch_i8<27>, dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, b->d_fhat,
+                           b->d_fexp, P.d_partial, b->d_counter);
+      else
+        hipLaunchKernelGGL(k_batch_i8<8>, dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, b->d_fhat,
+                           b->d_fexp, P.d_partial, b->d_counter);
+    } else if (D == 27) {
       if (b->precision == DGV_FP32_COMP) DGV_BATCH_LAUNCH(27, DGV_FP32_COMP);
       else if (b->precision == DGV_FP32_F64ACC) DGV_BATCH_LAUNCH(27, DGV_FP32_F64ACC);
       else DGV_BATCH_LAUNCH(27, DGV_FP64);
@@ -889,8 +1231,9 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     return DGV_ERR_ARG;
   }
   *out = nullptr;
-  if (precision != DGV_FP64 && precision != DGV_FP32_COMP && precision != DGV_FP32_F64ACC) {
-    set_global_error("precision must be DGV_FP64, DGV_FP32_COMP or DGV_FP32_F64ACC");
+  if (precision != DGV_FP64 && precision != DGV_FP32_COMP && precision != DGV_FP32_F64ACC &&
+      precision != DGV_INT8_SPLIT) {
+    set_global_error("precision must be DGV_FP64, DGV_FP32_COMP, DGV_FP32_F64ACC or DGV_INT8_SPLIT");
     return DGV_ERR_ARG;
   }
   const int D = grids[0] ? grids[0]->g.D : 0;
@@ -939,8 +1282,12 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     if (hipMalloc(&b->d_tmp[i], nb)) return fail(DGV_ERR_NOMEM, "hipMalloc(batch buffers)");
   if (hipMemsetAsync(b->d_f, 0, nb, b->stream) || hipMemsetAsync(b->d_q, 0, nb, b->stream))
     return fail(DGV_ERR_HIP, "hipMemset");
-  if (precision != DGV_FP64 && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
+  const bool fp32 = precision == DGV_FP32_COMP || precision == DGV_FP32_F64ACC;
+  const bool i8 = precision == DGV_INT8_SPLIT;
+  if (fp32 && hipMalloc(&b->d_f32, sizeof(float) * std::max<int64_t>(b->total, 1)))
     return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
+  if (i8 && (hipMalloc(&b->d_fhat, nb) || hipMalloc(&b->d_fexp, sizeof(int32_t) * std::max<int64_t>(b->total / D, 1))))
+    return fail(DGV_ERR_NOMEM, "hipMalloc(fixed-point buffers)");
   if (hipMalloc(&b->d_counter, sizeof(int) * 4)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
   {
     int64_t nmax = 1;
@@ -951,6 +1298,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
   std::vector<BatchGridDev> gd(ngrids);
   b->d_blist.assign(ngrids, nullptr);
   b->d_pairs32.assign(ngrids, nullptr);
+  b->d_i8.assign(ngrids, {nullptr, nullptr, nullptr, nullptr});
   b->cell_ptr.assign(ngrids, {});
   b->h_blist.assign(ngrids, {});
   for (int g = 0; g < ngrids; ++g) {
@@ -990,7 +1338,35 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
         hipMemcpy(b->d_blist[g], blist.data(), sizeof(int32_t) * blist.size(), hipMemcpyHostToDevice))
       return fail(DGV_ERR_NOMEM, "block lists");
     b->h_blist[g] = blist;
-    if (precision != DGV_FP64) {
+    if (i8) {
+      // blocks padded to multiples of 16 pairs, in block order
+      std::vector<int64_t> p16(pb.blocks.size());
+      int64_t tot16 = 0;
+      for (size_t k = 0; k < pb.blocks.size(); ++k) {
+        p16[k] = tot16;
+        tot16 += (pb.blocks[k].npairs + 15) & ~15;
+      }
+      const int RT = (D + 15) / 16;
+      const int64_t dw = (int64_t)RT * 32;   // digit dwords per pair
+      auto& a = b->d_i8[g];
+      if (hipMalloc(&a[0], sizeof(uint32_t) * std::max<int64_t>(tot16 * dw, 1)) ||
+          hipMalloc(&a[1], sizeof(uint32_t) * std::max<int64_t>(tot16, 1)) ||
+          hipMalloc(&a[2], sizeof(double) * 16 * RT * std::max<size_t>(pb.blocks.size(), 1)) ||
+          hipMalloc(&a[3], sizeof(int64_t) * std::max<size_t>(p16.size(), 1)))
+        return fail(DGV_ERR_NOMEM, "hipMalloc(INT8_SPLIT coefficient digits)");
+      if (hipMemcpy(a[3], p16.data(), sizeof(int64_t) * p16.size(), hipMemcpyHostToDevice))
+        return fail(DGV_ERR_HIP, "hipMemcpy(block starts)");
+      if (!pb.blocks.empty()) {
+        if (D == 27)
+          hipLaunchKernelGGL(k_i8_digits<27>, dim3((unsigned)pb.blocks.size()), dim3(256), 0, b->stream, pb.d_blocks,
+                             (const int64_t*)a[3], pb.d_pairs, (uint32_t*)a[0], (uint32_t*)a[1], (double*)a[2]);
+        else
+          hipLaunchKernelGGL(k_i8_digits<8>, dim3((unsigned)pb.blocks.size()), dim3(256), 0, b->stream, pb.d_blocks,
+                             (const int64_t*)a[3], pb.d_pairs, (uint32_t*)a[0], (uint32_t*)a[1], (double*)a[2]);
+        if (hipGetLastError() != hipSuccess) return fail(DGV_ERR_HIP, "INT8_SPLIT digit kernel launch");
+      }
+    }
+    if (fp32) {
       // zero pairs past the end: the kernel's load ring reads up to 4 kRing + 3
       // pairs beyond a block without clamping
       const int64_t n = pb.npairs * (D + 2), pad = (int64_t)kPadPairs * (D + 2);
@@ -1007,7 +1383,12 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     gd[g].blocks = pb.d_blocks;
     gd[g].blist = b->d_blist[g];
     gd[g].gwts = ctx->d_gwts;
+    gd[g].digits = (const uint32_t*)b->d_i8[g][0];
+    gd[g].codes = (const uint32_t*)b->d_i8[g][1];
+    gd[g].bscale = (const double*)b->d_i8[g][2];
+    gd[g].pbeg16 = (const int64_t*)b->d_i8[g][3];
     gd[g].foff = b->foff[g];
+    gd[g].eoff = b->foff[g] / D;
     gd[g].Bpad = b->Bpad[g];
     gd[g].Mv = ctx->g.Mv;
     gd[g].Mw = ctx->g.Mw;
@@ -1054,6 +1435,9 @@ void dgv_batch_destroy(dgv_batch* b) {
   for (int i = 0; i < 4; ++i) hipFree(b->d_tmp[i]);
   for (auto p : b->d_blist) hipFree(p);
   for (auto p : b->d_pairs32) hipFree(p);
+  for (auto& a : b->d_i8)
+    for (void* p : a) hipFree(p);
+  hipFree(b->d_fhat); hipFree(b->d_fexp);
   for (int i = 0; i < 3; ++i)
     if (b->ev[i]) hipEventDestroy(b->ev[i]);
   if (b->stream) hipStreamDestroy(b->stream);

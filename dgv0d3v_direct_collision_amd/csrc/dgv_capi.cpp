@@ -14,6 +14,8 @@ int launch_lincomb(dgv_ctx* ctx, int64_t n, double* out, int nterms, const doubl
                    const double* m2, const double* den, const int* neg);
 int get_basis_records(const dgv_ctx* ctx, int l, double* A, int32_t* xi, int32_t* xi1, int64_t cap,
                       int64_t* nrec, int index_base);
+int export_bases_ordered(const dgv_ctx* ctx, const int32_t* ls, int nl, double* out_A, int32_t* out_xi,
+                         int32_t* out_xi1, int index_base);
 int get_all_records(const dgv_ctx* ctx, std::vector<int32_t>& capphi, std::vector<double>& A,
                     std::vector<int32_t>& xi, std::vector<int32_t>& xi1, int index_base);
 int launch_eval_main_only(dgv_ctx* ctx, const double* d_f);
@@ -225,9 +227,29 @@ int dgv_ctx_get_aarray_basis(const dgv_ctx* ctx, int32_t l, double* A, int32_t* 
   return DGV_OK;
 }
 
-// Fortran unformatted sequential records: int32 length, payload, int32 length
-// (WriteAarraysDGV, readwrite.f90:1459-1484).  Payloads above 2^31-1 bytes
-// would need compiler-specific sub-records and are refused.
+int dgv_ctx_get_aarray_bases(const dgv_ctx* ctx, const int32_t* ls, int32_t nl, double* A, int32_t* A_xi,
+                             int32_t* A_xi1, int64_t cap, int64_t* counts, int index_base) {
+  CHECK_CTX(ctx);
+  dgv_ctx* c = const_cast<dgv_ctx*>(ctx);
+  if (!ctx->has_a) return set_error(c, DGV_ERR_STATE, "no A-array set");
+  if (!ls || nl < 1 || !counts || (index_base != 0 && index_base != 1))
+    return set_error(c, DGV_ERR_ARG, "bad basis list, counts pointer or index_base");
+  int64_t total = 0;
+  for (int k = 0; k < nl; ++k) {
+    if (ls[k] < 0 || ls[k] >= ctx->g.D) return set_error(c, DGV_ERR_INDEX, "basis index outside 0..D-1");
+    counts[k] = ctx->pb.nrec_basis[ls[k]];
+    total += counts[k];
+  }
+  if (!A && !A_xi && !A_xi1) return DGV_OK;
+  if (!A || !A_xi || !A_xi1) return set_error(c, DGV_ERR_ARG, "A, A_xi and A_xi1 are all needed");
+  if (cap < total) return set_error(c, DGV_ERR_ARG, "capacity smaller than the record count");
+  int rc = export_bases_ordered(ctx, ls, nl, A, A_xi, A_xi1, index_base);
+  if (rc) return set_error(c, rc, "A-array export failed");
+  return DGV_OK;
+}
+
+// WriteAarraysDGV / ReadAarraysDGV (readwrite.f90:1459-1484, 1066-1118): the
+// file layout and the framing of records above 2^31 - 9 bytes are dgv_aio.cpp's.
 int dgv_aarray_write_file(const dgv_ctx* ctx, const char* path) {
   CHECK_CTX(ctx);
   dgv_ctx* c = const_cast<dgv_ctx*>(ctx);
@@ -237,47 +259,24 @@ int dgv_aarray_write_file(const dgv_ctx* ctx, const char* path) {
   std::vector<double> A;
   if (get_all_records(ctx, capphi, A, xi, xi1, 1)) return set_error(c, DGV_ERR_HIP, "A-array export failed");
   const int64_t mm = (int64_t)capphi.size(), nn = (int64_t)A.size();
-  const int64_t bytes2 = 4 * mm + 8 * nn + 12 * nn;
-  if (bytes2 > INT32_MAX || mm > INT32_MAX || nn > INT32_MAX)
-    return set_error(c, DGV_ERR_UNSUPPORTED, "A-array record above 2 GiB needs Fortran sub-records");
-  std::ofstream o(path, std::ios::binary);
-  if (!o) return set_error(c, DGV_ERR_IO, std::string("cannot open ") + path);
-  auto w32 = [&](int32_t v) { o.write(reinterpret_cast<const char*>(&v), 4); };
-  w32(8); w32((int32_t)mm); w32((int32_t)nn); w32(8);
-  w32((int32_t)bytes2);
-  o.write(reinterpret_cast<const char*>(capphi.data()), 4 * mm);
-  o.write(reinterpret_cast<const char*>(A.data()), 8 * nn);
-  o.write(reinterpret_cast<const char*>(xi.data()), 4 * nn);
-  o.write(reinterpret_cast<const char*>(xi1.data()), 4 * nn);
-  // A_phi: the owning node of each record, 1-based
+  std::vector<int32_t> phi(nn);   // A_phi: the owning node of each record, 1-based
   for (int64_t k = 0, r = 0; k < mm; ++k)
-    for (int32_t j = 0; j < capphi[k]; ++j, ++r) w32((int32_t)(k + 1));
-  w32((int32_t)bytes2);
-  if (!o) return set_error(c, DGV_ERR_IO, "write failed");
+    for (int32_t j = 0; j < capphi[k]; ++j, ++r) phi[r] = (int32_t)(k + 1);
+  if (int rc = dgv_aarray_file_write(path, capphi.data(), mm, A.data(), xi.data(), xi1.data(), phi.data(), nn,
+                                     DGV_FRAMING_SUBRECORDS))
+    return set_error(c, rc, dgv_last_error(nullptr));
   return DGV_OK;
 }
 
 int dgv_ctx_read_aarray_file(dgv_ctx* ctx, const char* path) {
   CHECK_CTX(ctx);
   if (!path) return set_error(ctx, DGV_ERR_ARG, "null path");
-  std::ifstream in(path, std::ios::binary);
-  if (!in) return set_error(ctx, DGV_ERR_IO, std::string("cannot open ") + path);
-  auto r32 = [&](int32_t& v) { in.read(reinterpret_cast<char*>(&v), 4); return (bool)in; };
-  int32_t l1, mm, nn, l2;
-  if (!r32(l1) || !r32(mm) || !r32(nn) || !r32(l2) || l1 != 8 || l2 != 8 || mm < 0 || nn < 0)
-    return set_error(ctx, DGV_ERR_IO, "A-array file: bad header record");
-  int32_t len;
-  const int64_t want = 4 * (int64_t)mm + 20 * (int64_t)nn;
-  if (!r32(len) || len != want) return set_error(ctx, DGV_ERR_IO, "A-array file: bad data record length");
+  int64_t mm = 0, nn = 0;
+  if (int rc = dgv_aarray_file_header(path, &mm, &nn)) return set_error(ctx, rc, dgv_last_error(nullptr));
   std::vector<int32_t> capphi(mm), xi(nn), xi1(nn), phi(nn);
   std::vector<double> A(nn);
-  in.read(reinterpret_cast<char*>(capphi.data()), 4 * (int64_t)mm);
-  in.read(reinterpret_cast<char*>(A.data()), 8 * (int64_t)nn);
-  in.read(reinterpret_cast<char*>(xi.data()), 4 * (int64_t)nn);
-  in.read(reinterpret_cast<char*>(xi1.data()), 4 * (int64_t)nn);
-  in.read(reinterpret_cast<char*>(phi.data()), 4 * (int64_t)nn);
-  int32_t tail;
-  if (!in || !r32(tail) || tail != len) return set_error(ctx, DGV_ERR_IO, "A-array file truncated or corrupt");
+  if (int rc = dgv_aarray_file_read(path, capphi.data(), mm, A.data(), xi.data(), xi1.data(), phi.data(), nn))
+    return set_error(ctx, rc, dgv_last_error(nullptr));
   return build_pairblock_from_records(ctx, capphi.data(), mm, A.data(), xi.data(), xi1.data(), nn, 1);
 }
 

@@ -584,45 +584,70 @@ int ensure_record_layout(dgv_ctx* ctx) {
   return DGV_OK;
 }
 
-// Records of one basis function straight from the device layout (no cache),
-// for A-arrays too large to export whole.
-static int export_one_basis(const dgv_ctx* ctx, int l, std::vector<std::array<int64_t, 2>>& idx,
-                            std::vector<double>& val) {
+// Records of several basis functions straight from the device layout (no host
+// cache), for A-arrays too large to export whole.  The pair array comes to the
+// host once.  The reference order (xi ascending, then xi1) is produced by
+// construction, without a sort: xi = xc * D + p and xi1 = yc * D + q with
+// xc <= yc, so walking the blocks in (xc, yc) order, and for each xc the local
+// index p over every block of that xc before the next p, emits xi in order; the
+// pairs of one (xc, p) come from blocks of increasing yc, each sorted by q.
+// out_A / out_xi / out_xi1 receive the lists of ls[0], ls[1], ... back to back.
+int export_bases_ordered(const dgv_ctx* ctx, const int32_t* ls, int nl, double* out_A, int32_t* out_xi,
+                         int32_t* out_xi1, int index_base) {
   const Grid& g = ctx->g;
   const PairBlock& pb = ctx->pb;
   const int D = g.D;
-  idx.clear();
-  val.clear();
-  idx.reserve(pb.nrec_basis[l]);
-  val.reserve(pb.nrec_basis[l]);
-  std::vector<double> col;
-  for (const BlockHdr& B : pb.blocks) {
-    const int n = B.npairs;
-    col.resize((size_t)n * (D + 1));
-    if (n > 0 && hipMemcpy(col.data(), pb.d_pairs + B.pbeg * (D + 1), sizeof(double) * col.size(),
-                           hipMemcpyDeviceToHost) != hipSuccess)
-      return DGV_ERR_HIP;
-    const int64_t xc = ((int64_t)(g.ccu + B.a[0]) * g.Mv + (g.ccv + B.a[1])) * g.Mw + (g.ccw + B.a[2]);
-    const int64_t yc = ((int64_t)(g.ccu + B.b[0]) * g.Mv + (g.ccv + B.b[1])) * g.Mw + (g.ccw + B.b[2]);
-    for (int p = 0; p < D; ++p)
-      for (int j = B.seg[p]; j < B.seg[p + 1]; ++j) {
-        const double* R = &col[(size_t)j * (D + 1)];
-        if (R[l] == 0.0) continue;
-        int64_t q;
-        std::memcpy(&q, &R[D], sizeof(q));
-        q &= 255;
-        idx.push_back({xc * D + p, yc * D + q});
-        val.push_back(R[l]);
-      }
+  std::vector<double> pairs;
+  try {
+    pairs.resize((size_t)pb.npairs * (D + 1));
+  } catch (const std::bad_alloc&) {
+    return DGV_ERR_NOMEM;
   }
-  std::vector<size_t> o(idx.size());
-  std::iota(o.begin(), o.end(), 0);
-  std::sort(o.begin(), o.end(), [&](size_t x, size_t y) { return idx[x] < idx[y]; });
-  std::vector<std::array<int64_t, 2>> I2(o.size());
-  std::vector<double> V2(o.size());
-  for (size_t k = 0; k < o.size(); ++k) { I2[k] = idx[o[k]]; V2[k] = val[o[k]]; }
-  idx.swap(I2);
-  val.swap(V2);
+  if (pb.npairs > 0 &&
+      hipMemcpy(pairs.data(), pb.d_pairs, sizeof(double) * pairs.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return DGV_ERR_HIP;
+  auto cell_of = [&](const int8_t* o) {
+    return ((int64_t)(g.ccu + o[0]) * g.Mv + (g.ccv + o[1])) * g.Mw + (g.ccw + o[2]);
+  };
+  std::vector<int32_t> order(pb.blocks.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+    const int64_t ax = cell_of(pb.blocks[x].a), ay = cell_of(pb.blocks[y].a);
+    if (ax != ay) return ax < ay;
+    return cell_of(pb.blocks[x].b) < cell_of(pb.blocks[y].b);
+  });
+  std::vector<int64_t> pos(nl, 0);
+  for (int k = 1; k < nl; ++k) pos[k] = pos[k - 1] + pb.nrec_basis[ls[k - 1]];
+  std::vector<int64_t> end(nl);
+  for (int k = 0; k < nl; ++k) end[k] = pos[k] + pb.nrec_basis[ls[k]];
+  for (size_t b0 = 0; b0 < order.size();) {
+    const int64_t xc = cell_of(pb.blocks[order[b0]].a);
+    size_t b1 = b0;
+    while (b1 < order.size() && cell_of(pb.blocks[order[b1]].a) == xc) ++b1;
+    for (int p = 0; p < D; ++p)
+      for (size_t bi = b0; bi < b1; ++bi) {
+        const BlockHdr& B = pb.blocks[order[bi]];
+        const int64_t yc = cell_of(B.b);
+        for (int j = B.seg[p]; j < B.seg[p + 1]; ++j) {
+          const double* R = &pairs[(size_t)(B.pbeg + j) * (D + 1)];
+          int64_t q;
+          std::memcpy(&q, &R[D], sizeof(q));
+          q &= 255;
+          for (int k = 0; k < nl; ++k) {
+            const double a = R[ls[k]];
+            if (a == 0.0) continue;
+            if (pos[k] >= end[k]) return DGV_ERR_STATE;   // counts and layout disagree
+            out_A[pos[k]] = a;
+            out_xi[pos[k]] = (int32_t)(xc * D + p + index_base);
+            out_xi1[pos[k]] = (int32_t)(yc * D + q + index_base);
+            ++pos[k];
+          }
+        }
+      }
+    b0 = b1;
+  }
+  for (int k = 0; k < nl; ++k)
+    if (pos[k] != end[k]) return DGV_ERR_STATE;
   return DGV_OK;
 }
 
@@ -633,16 +658,11 @@ int get_basis_records(const dgv_ctx* ctx, int l, double* A, int32_t* xi, int32_t
     *nrec = ctx->pb.nrec_basis[l];
     if (!A && !xi && !xi1) return DGV_OK;
     if (cap < *nrec) return DGV_ERR_ARG;
-    std::vector<std::array<int64_t, 2>> I;
-    std::vector<double> V;
-    int rc = export_one_basis(ctx, l, I, V);
-    if (rc) return rc;
-    for (size_t k = 0; k < V.size(); ++k) {
-      if (A) A[k] = V[k];
-      if (xi) xi[k] = (int32_t)(I[k][0] + index_base);
-      if (xi1) xi1[k] = (int32_t)(I[k][1] + index_base);
-    }
-    return DGV_OK;
+    std::vector<double> tA(A ? 0 : *nrec);
+    std::vector<int32_t> tx(xi ? 0 : *nrec), tx1(xi1 ? 0 : *nrec);
+    const int32_t one[1] = {l};
+    return export_bases_ordered(ctx, one, 1, A ? A : tA.data(), xi ? xi : tx.data(), xi1 ? xi1 : tx1.data(),
+                                index_base);
   }
   const IdxT* pidx = nullptr;
   const ValT* pval = nullptr;

@@ -36,7 +36,7 @@ enum {
   DGV_ERR_UNSUPPORTED = -8
 };
 
-enum { DGV_FP64 = 0, DGV_FP32_COMP = 1, DGV_FP32_F64ACC = 2 };   /* precision */
+enum { DGV_FP64 = 0, DGV_FP32_COMP = 1, DGV_FP32_F64ACC = 2, DGV_INT8_SPLIT = 3 };   /* precision */
 enum { DGV_KERNEL_AUTO = 0, DGV_KERNEL_PAIRBLOCK = 1, DGV_KERNEL_RECORD = 2, DGV_KERNEL_MFMA = 3 };
 
 typedef struct dgv_ctx dgv_ctx;
@@ -109,10 +109,58 @@ int dgv_ctx_workload(const dgv_ctx* ctx, int64_t out[4]);
  * in the reference's serial order (xi ascending, then xi1); index_base 0/1 */
 int dgv_ctx_get_aarray_basis(const dgv_ctx* ctx, int32_t l, double* A, int32_t* A_xi,
                              int32_t* A_xi1, int64_t cap, int64_t* nrec, int index_base);
-/* Fortran-unformatted chunk file of WriteAarraysDGV / ReadAarraysDGV
- * (readwrite.f90:1066-1118,1459-1484). */
+/* the records of the canonical basis functions ls[0..nl-1], each list in the
+ * reference's serial order, back to back in A / A_xi / A_xi1 (counts[k] records
+ * for ls[k]); one pass over the device layout (no host cache).  With A, A_xi and
+ * A_xi1 all NULL only the counts are returned.  Same readers as above
+ * (ReadAarraysDGV's A, A_xi, A_xi1 arrays, readwrite.f90:1329-1333). */
+int dgv_ctx_get_aarray_bases(const dgv_ctx* ctx, const int32_t* ls, int32_t nl, double* A,
+                             int32_t* A_xi, int32_t* A_xi1, int64_t cap, int64_t* counts,
+                             int index_base);
+/* ---- A-array files (readwrite.f90:1066-1484, 2224-2365) ---------------------
+ * Fortran unformatted sequential: record 1 = (mm, nn) int32; record 2 =
+ * A_capphi(mm) int32, A(nn) float64, A_xi(nn), A_xi1(nn), A_phi(nn) int32.
+ * Records above 2^31 - 9 bytes (every grid from 21^3 on) are framed either as
+ * Intel Fortran / gfortran subrecords or as LLVM Flang's single record with the
+ * length modulo 2^32; readers recognise both, writers take the framing. */
+enum { DGV_FRAMING_SUBRECORDS = 0, DGV_FRAMING_UNSIGNED32 = 1 };
+/* WriteAarraysDGV / ReadAarraysDGV of the context's A-array (readwrite.f90:
+ * 1459-1484, 1066-1118); the writer uses DGV_FRAMING_SUBRECORDS. */
 int dgv_aarray_write_file(const dgv_ctx* ctx, const char* path);
 int dgv_ctx_read_aarray_file(dgv_ctx* ctx, const char* path);
+/* the same file on host arrays (no GPU): sizes of record 1, record 2 read into
+ * caller arrays of exactly those sizes, and the writer */
+int dgv_aarray_file_header(const char* path, int64_t* mm, int64_t* nn);
+int dgv_aarray_file_read(const char* path, int32_t* capphi, int64_t mm, double* A, int32_t* A_xi,
+                         int32_t* A_xi1, int32_t* A_phi, int64_t nn);
+int dgv_aarray_file_write(const char* path, const int32_t* capphi, int64_t mm, const double* A,
+                          const int32_t* A_xi, const int32_t* A_xi1, const int32_t* A_phi, int64_t nn,
+                          int32_t framing);
+/* test hook: largest subrecord payload of DGV_FRAMING_SUBRECORDS (default and
+ * maximum 2^31 - 9 bytes), so that the framing is exercised on small files */
+int dgv_set_fortran_subrecord_limit(int64_t bytes);
+/* chunk file names: dgv_aarray_chunk_name below */
+/* ScanAarrsChnks4Acapphi (readwrite.f90:2292-2365): records in each of chunks
+ * 0 .. nchnks-1 (sum of the chunk's A_capphi) */
+int dgv_aarray_chunks_scan(const dgv_grid_desc* grid, const char* dir, const char* base, int32_t k_c,
+                           int32_t N_x, int32_t nchnks, int64_t* chunks_Act);
+/* ReadAarraysChnksOfstNrecDGV(fstchnk, ofst, numrec, nchnks) (readwrite.f90:
+ * 1261-1449): numrec records from chunk fstchnk (1-based) on, the first ofst
+ * records of that chunk skipped; capphi[mm] gets the reference's per-node
+ * counts of the records read; *nread = records read.  ReadAarraysChnksDGV
+ * (readwrite.f90:1130-1243) is fstchnk = 1, ofst = 0, numrec = all.  Where the
+ * reference prints and stops, the call returns DGV_ERR_INDEX or DGV_ERR_IO. */
+int dgv_aarray_chunks_read(const dgv_grid_desc* grid, const char* dir, const char* base, int32_t k_c,
+                           int32_t N_x, int32_t fstchnk, int64_t ofst, int64_t numrec, int32_t nchnks,
+                           int32_t* capphi, int64_t mm_cap, double* A, int32_t* A_xi, int32_t* A_xi1,
+                           int32_t* A_phi, int64_t nrec_cap, int64_t* nread);
+/* the context's A-array as a chunk set of at most max_records records per chunk
+ * (the reference's chunk files come from its MPI generator; no single routine) */
+int dgv_ctx_write_aarray_chunks(const dgv_ctx* ctx, const char* dir, const char* base, int32_t k_c,
+                                int32_t N_x, int64_t max_records, int32_t framing, int32_t* nchunks);
+/* ReadAarraysChnksDGV(nchnks) into the context */
+int dgv_ctx_read_aarray_chunks(dgv_ctx* ctx, const char* dir, const char* base, int32_t k_c, int32_t N_x,
+                               int32_t nchnks);
 /* TruncAarrsTrhls_DGV(trhld) (dgvtools_mod.f90:1313-1362): keeps the records with
  * |A| > trhld.  TruncAarrsRadius_DGV(rad) (dgvtools_mod.f90:1375-1429): keeps
  * the records whose nodes xi, xi1 lie at most rad apart.  Both work on the
@@ -251,7 +299,11 @@ int dgv_project_invariants_dev(dgv_ctx* ctx, double* d_q, int64_t n);
 typedef struct dgv_batch dgv_batch;
 /* precision: DGV_FP64; DGV_FP32_COMP (FP32 products and matrix-core sums over
  * 32 pairs, two-sum compensated FP32 accumulation, FP64 final sums);
- * DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in FP64) */
+ * DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in FP64);
+ * DGV_INT8_SPLIT (coefficients as 38-bit and products f f as 30-bit fixed-point
+ * integers, per block and per cell scaled, split into signed base-256 digits;
+ * the digit products with weight >= 256^-4 of the leading one are summed exactly
+ * on the int8 matrix cores in int32, per block, then scaled and added in FP64) */
 int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_of_instance,
                      int32_t ninstances, int32_t precision, dgv_batch** out);
 /* A batch reads its grids' A-arrays: once one of them is replaced, truncated or

@@ -14,7 +14,10 @@ function               reference routine restated
                        (gaussian_mod.f90:4-53, miscset.f90:29-1311)
 ``eval_aplus``         EvalCollisionPeriodicAPlus_DGV (collision_mod.f90:143-229)
 ``eval_a``             EvalCollisionPeriodicA_DGV (collision_mod.f90:29-129)
+``eval_aplus_list``    the same loop at a list of nodes (bench's CPU sample, 27^3 parity)
 ``eval_aplus_comp``    the same loop with Neumaier-compensated accumulation
+``eval_aplus_dd_list`` the same loop at listed nodes, exact products and a
+                       double-double sum (Q rounded once: the conditioning-free checker)
 ``iphi``               IphiDGV (dgvtools_mod.f90:847-877)
 ``masscheck``          MassCheckRec (dgvtools_mod.f90:1437-1458)
 ``masscheck_high``     MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549)
@@ -29,6 +32,8 @@ function               reference routine restated
                        SpatialOperSH_DGV (spat_oper_mod.f90:195-272, 28-68)
 ``trunc_threshold``     TruncAarrsTrhls_DGV (dgvtools_mod.f90:1313-1362)
 ``trunc_radius``        TruncAarrsRadius_DGV (dgvtools_mod.f90:1375-1429)
+``read_chunks_ofst_nrec`` ReadAarraysChnksOfstNrecDGV (readwrite.f90:1261-1449) on
+                       in-memory chunks
 ``seta`` / ``seta_raw``  SetA_DGV with the Lagrange basis restored, its adaptive
                        Simpson integration in chi and epsilon (dgvtools_mod.f90:377-681,
                        938-1122, 232-289)
@@ -109,6 +114,8 @@ def lib():
                                           _f64p, _i64p, C.c_int64, C.c_int]
         L.orc_eval_aplus_comp.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p,
                                           _f64p, C.c_void_p, C.c_int64, C.c_int64, C.c_int]
+        L.orc_eval_aplus_dd_list.argtypes = [C.POINTER(_Geom), _i32p, _f64p, _i32p, _i32p, _f64p,
+                                             _f64p, _f64p, _i64p, C.c_int64, C.c_int]
         L.orc_iphi.argtypes = [C.c_int64, _i32p, _f64p, _i32p, _i32p, _f64p, _f64p,
                                C.c_double, _f64p]
         L.orc_masscheck.argtypes = [C.POINTER(_Geom), _f64p, _f64p]
@@ -222,6 +229,18 @@ def eval_aplus_list(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes, nthreads=1):
                               _c(A_xi1, np.int32), _c(f, np.float64), fc, nodes, len(nodes),
                               nthreads)
     return fc
+
+
+def eval_aplus_dd_list(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes, nthreads=1):
+    """a1 at the given 0-based nodes with exact products and a double-double
+    sum (Q rounded once); returns (Q, sum|terms|/w) at those nodes."""
+    nodes = _c(nodes, np.int64)
+    fc = np.zeros(len(nodes))
+    ab = np.zeros(len(nodes))
+    lib().orc_eval_aplus_dd_list(G.ptr, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
+                                 _c(A_xi1, np.int32), _c(f, np.float64), fc, ab, nodes, len(nodes),
+                                 nthreads)
+    return fc, ab
 
 
 def eval_a(G: Geometry, capphi, A, A_xi, A_xi1, f, nodes=None, nthreads=1):
@@ -446,3 +465,70 @@ class Regime:
         if self.run_mode == 1:
             return eval_aplus(self.G, *self.A, df, nthreads=8) + lin
         return lin
+
+
+def read_chunks_ofst_nrec(chunks, fstchnk, ofst, numrec):
+    """ReadAarraysChnksOfstNrecDGV (readwrite.f90:1261-1449) on in-memory chunks:
+    chunks[j] = (A_capphi, A, A_xi, A_xi1, A_phi) of chunk j (0-based list, the
+    routine's chunk numbers are 1-based).  Returns the arrays as the routine
+    leaves them in commvar: (A_capphi(mm), A, A_xi, A_xi1, A_phi) of length
+    numrec (shorter if the chunks run out).  Raises where it prints and stops.
+    Plain loops with the routine's 1-based indices."""
+    nchnks = len(chunks)
+    cap1, a1, x1, y1, p1 = chunks[fstchnk - 1]
+    mm, nn = len(cap1), len(a1)
+    A_capphi = [0] * (mm + 1)                        # 1-based
+    A, A_xi, A_xi1, A_phi = [], [], [], []
+    Act = 0
+    Act_need = min(nn - ofst, numrec - Act)
+    A += list(a1[ofst:ofst + Act_need])
+    A_xi += list(x1[ofst:ofst + Act_need])
+    A_xi1 += list(y1[ofst:ofst + Act_need])
+    A_phi += list(p1[ofst:ofst + Act_need])
+    Act = Act_need
+    csum = lambda i: int(sum(cap1[:i]))              # sum(Achnks_capphi(1:i))
+    i = 1
+    while i <= mm and ofst > csum(i):
+        i = i + 1
+    if i > mm:
+        raise IndexError("no records in the chunk after offset taken")
+    if csum(i) - ofst < Act_need:
+        A_capphi[i] = csum(i) - ofst
+        Act_need = Act_need - csum(i) + ofst
+    else:
+        A_capphi[i] = Act_need
+        Act_need = 0
+    while Act_need > 0 and i < mm:
+        i = i + 1
+        if cap1[i - 1] >= Act_need:
+            A_capphi[i] = Act_need
+            Act_need = 0
+        else:
+            A_capphi[i] = int(cap1[i - 1])
+            Act_need = Act_need - int(cap1[i - 1])
+    if i >= mm and Act_need > 0:
+        raise IndexError("no records in the chunk after offset taken")
+    j = fstchnk + 1
+    while Act < numrec and j <= nchnks:
+        capj, aj, xj, yj, pj = chunks[j - 1]
+        nn = len(aj)
+        Act_need = min(nn, numrec - Act)
+        A += list(aj[:Act_need])
+        A_xi += list(xj[:Act_need])
+        A_xi1 += list(yj[:Act_need])
+        A_phi += list(pj[:Act_need])
+        Act = Act + Act_need
+        i = 0
+        while Act_need > 0 and i < mm:
+            i = i + 1
+            if capj[i - 1] >= Act_need:
+                A_capphi[i] = A_capphi[i] + Act_need
+                Act_need = 0
+            else:
+                A_capphi[i] = A_capphi[i] + int(capj[i - 1])
+                Act_need = Act_need - int(capj[i - 1])
+        if i >= mm and Act_need > 0:
+            raise IndexError("no records in the chunk after offset taken")
+        j = j + 1
+    return (np.array(A_capphi[1:], np.int32), np.array(A, np.float64), np.array(A_xi, np.int32),
+            np.array(A_xi1, np.int32), np.array(A_phi, np.int32))

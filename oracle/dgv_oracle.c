@@ -418,6 +418,57 @@ int orc_eval_aplus_comp(const orc_geom* G, const int32_t* capphi, const double* 
   return 0;
 }
 
+/* a1 at a list of nodes with every term and the sum carried in double-double:
+ * each product f(xi') f(xi1') A is formed exactly (two-product with fma), and
+ * the terms are added with two-sum into a (hi, lo) pair, so the result is Q
+ * rounded once, whatever the conditioning.  The checker for kernels whose
+ * distance from the reference's sequential FP64 sum is dominated by that sum's
+ * own rounding.  abs_sum[t] = sum |terms| / w (the terms' scale). */
+static inline void two_prod(double a, double b, double* p, double* e) {
+  *p = a * b;
+  *e = fma(a, b, -*p);
+}
+static inline void two_sum_dd(double a, double b, double* s, double* e) {
+  *s = a + b;
+  double bp = *s - a;
+  *e = (a - (*s - bp)) + (b - bp);
+}
+int orc_eval_aplus_dd_list(const orc_geom* G, const int32_t* capphi, const double* A,
+                           const int32_t* A_xi, const int32_t* A_xi1, const double* f, double* fc,
+                           double* abs_sum, const int64_t* nodes, int64_t nsel, int nthreads) {
+  int64_t* ashift = (int64_t*)malloc(sizeof(int64_t) * G->nnodes);
+  orc_ashift(G, capphi, ashift);
+  int nt = set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int64_t t = 0; t < nsel; t++) {
+    const int64_t iphi = nodes[t];
+    int64_t Ashift = ashift[iphi];
+    int32_t phicap = capphi[G->nodes_phican[iphi] - 1];
+    int dui = G->nodes_dui[iphi], dvi = G->nodes_dvi[iphi], dwi = G->nodes_dwi[iphi];
+    double sh = 0.0, sl = 0.0, sa = 0.0;
+    for (int32_t j = 1; j <= phicap; j++) {
+      int64_t k1, k2;
+      if (shift_record(G, A_xi[Ashift + j - 1], A_xi1[Ashift + j - 1], dui, dvi, dwi, &k1, &k2)) {
+        double ph, pl, th, tl, s, e;
+        two_prod(f[k1 - 1], f[k2 - 1], &ph, &pl);          /* f f = ph + pl exactly */
+        two_prod(ph, A[Ashift + j - 1], &th, &tl);           /* ph A = th + tl exactly */
+        tl += pl * A[Ashift + j - 1];                        /* + pl A (below 2^-106 relative) */
+        two_sum_dd(sh, th, &s, &e);
+        sh = s;
+        sl += e + tl;
+        sa += fabs(th);
+      }
+    }
+    /* factor 2 is exact; (sh + sl) / w rounded once */
+    double q = 2.0 * sh, ql = 2.0 * sl;
+    double qh = q + ql;
+    fc[t] = (qh + ((q - qh) + ql)) / G->nodes_gwts[iphi];
+    if (abs_sum) abs_sum[t] = 2.0 * sa / G->nodes_gwts[iphi];
+  }
+  free(ashift);
+  return 0;
+}
+
 int orc_iphi(int64_t nnodes, const int32_t* capphi, const double* A, const int32_t* A_xi,
              const int32_t* A_xi1, const double* gwts, const double* f, double coef,
              double* out) {

@@ -63,6 +63,10 @@ int orc_eval_a(const orc_geom* G, const int32_t* capphi, const double* A,
                const int32_t* A_xi, const int32_t* A_xi1, const double* f,
                double* fc, int64_t node_begin, int64_t node_end, int nthreads);
 /* a1 with Neumaier-compensated accumulation: estimates a1's own rounding.      */
+/* a1 at listed nodes in double-double (exact products, two-sum accumulation) */
+int orc_eval_aplus_dd_list(const orc_geom* G, const int32_t* capphi, const double* A,
+                           const int32_t* A_xi, const int32_t* A_xi1, const double* f, double* fc,
+                           double* abs_sum, const int64_t* nodes, int64_t nsel, int nthreads);
 int orc_eval_aplus_comp(const orc_geom* G, const int32_t* capphi, const double* A,
                         const int32_t* A_xi, const int32_t* A_xi1, const double* f,
                         double* fc, double* abs_sum, int64_t node_begin,

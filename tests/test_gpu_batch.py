@@ -1,8 +1,11 @@
 """Batched evaluation (many VDFs, mixed grids, one persistent launch) against
 the single-VDF evaluator, whose parity with the oracle is tested elsewhere.
-FP64: relative L2 <= 1e-12.  FP32 products (two-sum compensated FP32 sums, or
-FP64 sums of the FP32 partial sums): error <= 4 * 2^-24 of the summed terms'
-scale and median relative L2 <= 1e-6 per VDF."""
+FP64: relative L2 <= 1e-12 for every VDF.  FP32 products (two-sum compensated
+FP32 sums, or FP64 sums of the FP32 partial sums): error <= 4 * 2^-24 of the
+summed terms' scale for every VDF, i.e. relative L2 <= 4 * 2^-24 * kappa with
+kappa = ||sum |A f f| || / ||Q||; a VDF with kappa above about 17 can miss 1e-6,
+and the test reports how many do.  INT8_SPLIT (exact integer digit products):
+error <= 2^-28 of the terms' scale and relative L2 <= 1e-6 for every VDF."""
 import numpy as np
 import pytest
 
@@ -50,8 +53,8 @@ def absq(grids):
         c.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("FP64", 1e-12), ("FP32_COMP", 1e-6), ("FP32_F64ACC", 1e-6)])
-def test_mixed_grid_batch_matches_single(grids, absq, precision, tol):
+@pytest.mark.parametrize("precision", ["FP64", "FP32_COMP", "FP32_F64ACC", "INT8_SPLIT"])
+def test_mixed_grid_batch_matches_single(grids, absq, precision):
     import dgv0d3v_direct_collision_amd as dgv
     rng = np.random.default_rng(11)
     which = np.array([0] * 70 + [1] * 10 + [0] * 3, np.int32)   # uneven, interleaved
@@ -63,30 +66,41 @@ def test_mixed_grid_batch_matches_single(grids, absq, precision, tol):
         fs.append(f)
         b.set_f(i, f)
     b.eval_q()
-    errs, scaled = [], []
+    errs, scaled, kappa = [], [], []
     for i, g in enumerate(which):
         ref = grids[g].EvalCollisionPeriodicAPlus_DGV(fs[i])
         q = b.get_q(i)
         errs.append(rel_l2(q, ref))
         if precision != "FP64":   # error against the scale of the summed terms
-            scaled.append(np.linalg.norm(q - ref) / np.linalg.norm(absq[g](fs[i])))
-    print(f"[batch] {precision}: max rel L2 {max(errs):.3e}", flush=True)
+            qa = np.linalg.norm(absq[g](fs[i]))
+            scaled.append(np.linalg.norm(q - ref) / qa)
+            kappa.append(qa / np.linalg.norm(ref))
+    errs = np.array(errs)
+    print(f"[batch] {precision}: max rel L2 {errs.max():.3e}, {np.mean(errs <= 1e-6):.3f} of VDFs within 1e-6",
+          flush=True)
     if precision == "FP64":
-        assert max(errs) <= tol
+        assert errs.max() <= 1e-12
+    elif precision == "INT8_SPLIT":
+        print(f"[batch] {precision}: max ||dq|| / ||sum |A f f| || = {max(scaled):.3e} "
+              f"({max(scaled) * 2 ** 30:.3f} x 2^-30)", flush=True)
+        assert max(scaled) <= 2.0 ** -28
+        assert errs.max() <= 1e-6
     else:
         # FP32 products carry a relative error of a few 2^-24 each; after the
         # compensated sums the error is that times the terms' scale, not Q's
-        print(f"[batch] {precision}: max ||dq|| / ||sum |A f f| || = {max(scaled):.3e}", flush=True)
+        print(f"[batch] {precision}: max ||dq|| / ||sum |A f f| || = {max(scaled):.3e}; kappa max {max(kappa):.1f}",
+              flush=True)
         assert max(scaled) <= 4 * 2.0 ** -24
-        assert np.median(errs) <= tol
+        assert np.all(errs <= 4 * 2.0 ** -24 * np.array(kappa))
     b.close()
 
 
-def test_batch_rk_graph_matches_single(grids):
+@pytest.mark.parametrize("precision,tol", [("FP64", 1e-12), ("INT8_SPLIT", 1e-8)])
+def test_batch_rk_graph_matches_single(grids, precision, tol):
     import dgv0d3v_direct_collision_amd as dgv
     rng = np.random.default_rng(5)
     which = np.array([0, 1, 0, 1, 1], np.int32)
-    b = dgv.DGVBatch(grids, which, dgv.DGV_FP64)
+    b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + precision))
     fs = [vdf(grids[g], rng, 2) for g in which]
     for i, f in enumerate(fs):
         b.set_f(i, f)
@@ -97,7 +111,7 @@ def test_batch_rk_graph_matches_single(grids):
         grids[g].rkmts, grids[g].coef = 2, 1.0
         for _ in range(3):
             grids[g].TimeIntegratorORK_SH_DGV(ref, 0.02)
-        assert rel_l2(b.get_f(i), ref) <= 1e-12
+        assert rel_l2(b.get_f(i), ref) <= tol
     b.close()
 
 
@@ -185,7 +199,8 @@ def test_batch_device_rows_match_per_instance_copies(grids):
     b.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("FP64", 1e-12), ("FP32_COMP", 1e-5), ("FP32_F64ACC", 1e-5)])
+@pytest.mark.parametrize("precision,tol", [("FP64", 1e-12), ("FP32_COMP", 1e-5), ("FP32_F64ACC", 1e-5),
+                                           ("INT8_SPLIT", 1e-6)])
 def test_two_nodes_per_dimension_batch_matches_single(precision, tol):
     """D = 8 (s = 2) grids, synthetic A-arrays with uneven blocks: the batched
     kernels' D = 8 instantiation against the single-VDF evaluator."""
