@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdgv_amd.so")
+# DGV_LIB_PATH selects another build of the same library (the sanitizer test's
+# host-instrumented copy); there is no other compute path
+LIB_PATH = os.environ.get("DGV_LIB_PATH") or os.path.join(HERE, "libdgv_amd.so")
 
 DGV_OK = 0
 ERRORS = {-1: "DGV_ERR_ARG", -2: "DGV_ERR_GRID", -3: "DGV_ERR_INDEX", -4: "DGV_ERR_NOMEM",

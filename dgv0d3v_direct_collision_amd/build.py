@@ -65,6 +65,35 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+SANITIZE_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def build_sanitized(outdir: str, verbose: bool = False) -> str:
+    """A copy of libdgv_amd.so whose host code is built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (device code unchanged), into outdir.  Loaded by
+    the host-side tests through DGV_LIB_PATH with the sanitizer runtime preloaded."""
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+    os.makedirs(outdir, exist_ok=True)
+    procs, objs = [], []
+    for s in srcs:
+        o = os.path.join(outdir, os.path.basename(s) + ".o")
+        cmd = [hipcc(), *HIPFLAGS, *FILE_FLAGS.get(os.path.basename(s), []), *SANITIZE_HOST, "-g", "-x", "hip", "-c",
+               s, "-o", o]
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(o)
+    failed = [(s, p.communicate()[0].decode(errors="replace")) for s, p in procs]
+    failed = [(s, out) for (s, out), (_, p) in zip(failed, procs) if p.returncode != 0]
+    if failed:
+        for s, out in failed:
+            sys.stderr.write(f"--- {s}\n{out}\n")
+        raise RuntimeError("sanitized HIP compilation failed")
+    lib = os.path.join(outdir, "libdgv_amd.so")
+    subprocess.run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *SANITIZE_HOST, "-o", lib, *objs],
+                   check=True)
+    return lib
+
+
 def amdflang():
     for c in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
         if c and os.path.exists(c):

@@ -556,8 +556,9 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 //   level 3: A dword (A_0, A_1, A_2, A_3) against the natural x dword;
 //   level 4: A dword (A_1, A_2, A_3, A_4) against the natural x dword;
 //   level k < 3: the first A dword against the x dword shifted right by 8 (3 - k).
-// At the end of a block the five int32 sums are combined in FP64 and scaled by
-// 2^(eA - 38) 2^(e(c+a) + e(c+b) - 30) into the item's FP64 running sum.
+// At the end of a block the five int32 sums are combined in FP64 as
+// sum_k acc_k 256^(4-k) and scaled by 256^3 2^(eA - 38) 2^(e(c+a) + e(c+b) - 30)
+// into the item's FP64 running sum.
 // Dropped: digit products below 256^-4 of the leading one and the roundings of
 // At and x, i.e. relative errors near 2^-30 per term at the block's scale.
 typedef int bi4 __attribute__((ext_vector_type(4)));
@@ -602,7 +603,9 @@ __global__ __launch_bounds__(256) void k_i8_digits(const BlockHdr* __restrict__ 
   __syncthreads();
   if (threadIdx.x < 16 * RT) {
     const int l = threadIdx.x;
-    bscale[(int64_t)blk * 16 * RT + l] = (l < D && emax[l] != INT_MIN) ? ldexp(1.0, emax[l] - kI8BA) : 0.0;
+    // 2^(eA - 38) times 256^3, the weight of the lowest kept level (k = 4):
+    // the kernel combines the level sums as sum_k acc_k 256^(4 - k)
+    bscale[(int64_t)blk * 16 * RT + l] = (l < D && emax[l] != INT_MIN) ? ldexp(1.0, emax[l] - kI8BA + 24) : 0.0;
   }
   uint32_t* dblk = digits + o16 * i8_dwords_per_pair<D>();
   for (int i = threadIdx.x; i < n16 * 16 * RT; i += blockDim.x) {

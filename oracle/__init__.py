@@ -54,7 +54,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libdgv_oracle.so")
+# DGV_ORACLE_LIB: a sanitizer-instrumented build of the same sources (tests only)
+_LIB_PATH = os.environ.get("DGV_ORACLE_LIB") or os.path.join(_HERE, "libdgv_oracle.so")
 _lib = None
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
