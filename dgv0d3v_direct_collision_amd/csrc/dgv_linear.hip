@@ -583,6 +583,11 @@ int dgv_ctx_get_fma(dgv_ctx* ctx, double* fmA, int64_t n) {
 int dgv_ctx_set_regime(dgv_ctx* ctx, int32_t run_mode, double decomp_lev, double linear_lev) {
   if (!ctx) return DGV_ERR_ARG;
   if (run_mode < 0 || run_mode > 2) return set_error(ctx, DGV_ERR_ARG, "run_mode must be 0, 1 or 2");
+  // spat_oper_mod.f90:331-334: the decomposition threshold must lie above the
+  // linearisation threshold (decomp_lev = 0 switches the regime check off)
+  if (decomp_lev > 0 && decomp_lev <= linear_lev)
+    return set_error(ctx, DGV_ERR_ARG,
+                     "Decomposition treshold is lower than linearization treshhold (decomp_lev <= linear_lev)");
   if ((run_mode != 0) && !ctx->lin.fma_valid)
     return set_error(ctx, DGV_ERR_STATE, "run_mode 1 or 2 needs a linearised operator (PrepareFMA)");
   ctx->lin.run_mode = run_mode;

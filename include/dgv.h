@@ -219,7 +219,9 @@ int dgv_ctx_get_fm(dgv_ctx* ctx, double* fm, int64_t n);
  * turns on CheckSolutionMode_DGV before every spatial operator, as the
  * reference's SpatialOperSH_DGV calls it; decomp_lev <= 0 keeps run_mode 0 (the
  * reference's check then always lands in run_mode 0).  run_mode 1 or 2 needs a
- * prepared fmA.  With switching on, RK runs step eagerly (no graph replay). */
+ * prepared fmA.  With switching on, RK runs step eagerly (no graph replay).
+ * decomp_lev > 0 with decomp_lev <= linear_lev is DGV_ERR_ARG, where the
+ * reference prints and stops (spat_oper_mod.f90:331-334). */
 int dgv_ctx_set_regime(dgv_ctx* ctx, int32_t run_mode, double decomp_lev, double linear_lev);
 /* current run_mode, the last relative L1 deviation from the local Maxwellian
  * (the value KeepTrackL1_err_DGV records), the last |fm - f_M|_1 / n, and
