@@ -18,6 +18,9 @@ static thread_local std::string g_last_error;
 int set_error(dgv_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   g_last_error = msg;
+  // a failed hipMalloc also leaves HIP's sticky last error set: consume it here,
+  // where it is reported, so that the next launch check does not blame it
+  if (code == DGV_ERR_NOMEM) (void)hipGetLastError();
   return code;
 }
 void set_global_error(const std::string& msg) { g_last_error = msg; }

@@ -35,7 +35,7 @@ def test_batch_refuses_a_changed_aarray(mutation):
                 b.set_f(i, f)
             b.eval_q()
             q0 = b.get_q(0)
-            assert rel_l2(q0, g.EvalCollisionPeriodicAPlus_DGV(f)) <= (1e-4 if prec == "DGV_FP32_COMP" else 1e-10)
+            assert rel_l2(q0, g.EvalCollisionPeriodicAPlus_DGV(f)) <= (1e-12 if prec == "DGV_FP64" else 1e-6)
             if mutation == "truncate":
                 g.TruncAarrsTrhls_DGV(1e-6 if prec == "DGV_FP64" else 1e-5)
             elif mutation == "abs":
