@@ -88,6 +88,7 @@ SIGNATURES = {
                                         _i32, C.c_char_p, C.c_size_t]),
     "dgv_eval_q": (C.c_int, [_vp, _dp, _dp, _i64]),
     "dgv_eval_q_dev": (C.c_int, [_vp, _vp, _vp, _i64]),
+    "dgv_eval_q_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _i64]),
     "dgv_eval_q_a": (C.c_int, [_vp, _dp, _dp, _i64]),
     "dgv_eval_mixed": (C.c_int, [_vp, _dp, _dp, _dp, _i64]),
     "dgv_eval_mixed_dev": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),

@@ -84,6 +84,8 @@ int dgv_ctx_create(const dgv_grid_desc* grid, int device, dgv_ctx** out) {
   const Grid& g = ctx->g;
   auto fail = [&](int code, const char* m) { set_global_error(m); dgv_ctx_destroy(ctx); return code; };
   if (hipSetDevice(device) != hipSuccess) return fail(DGV_ERR_HIP, "hipSetDevice failed");
+  // the allocation-free evaluator's device array: its address on this device, once
+  if (static_scratch_setup(device) != DGV_OK) return fail(DGV_ERR_HIP, "hipGetSymbolAddress(scratch) failed");
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(DGV_ERR_HIP, "hipStreamCreate failed");
   for (int i = 0; i < 4; ++i)

@@ -256,7 +256,10 @@ int launch_scale(dgv_ctx* ctx, int64_t n, double* x, double s);
 int device_moments(dgv_ctx* ctx, const double* d_f, double out[5]);
 int device_collision_moments(dgv_ctx* ctx, const double* d_q, double out[5]);
 int device_moments_high(dgv_ctx* ctx, const double* d_f, double out[20]);
-int device_project(dgv_ctx* ctx, double* d_q);
+int device_project(dgv_ctx* ctx, double* d_q, double* scratch = nullptr);
+// the file-scope device scratch of dgv_eval_q_dev_noalloc, looked up once per
+// device at context creation (dgv_kernels.hip)
+int static_scratch_setup(int device);
 // Cholesky factor (row-major, lower) of the Gram matrix of psi = 1,u,v,w,|u|^2
 // in the w-weighted inner product (dgv_capi.cpp)
 int gram_cholesky(const Grid& g, double L[25]);

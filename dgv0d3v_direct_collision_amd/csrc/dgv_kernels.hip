@@ -1,5 +1,7 @@
 // dgv_kernels.hip -- gfx950 kernels of the direct DGV collision evaluator (FP64)
 // and the device arithmetic of the spatial operator and the time steppers.
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <algorithm>
 #include <cstring>
@@ -415,10 +417,13 @@ __global__ __launch_bounds__(256) void k_rec_eval(int D, int Mu, int Mv, int Mw,
 }
 
 // d_g != nullptr: the mixed term with g (EvalCollisionPeriodicMixedTermsA_DGV)
-static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, const double* d_g = nullptr) {
+// partial: where the items' partial rows go (the plan's buffer unless given)
+static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, const double* d_g = nullptr,
+                                 double* partial = nullptr) {
   const Grid& g = ctx->g;
   const PairBlock& pb = ctx->pb;
   const Plan& P = ctx->plan;
+  double* const part = partial ? partial : P.d_partial;
   const int nitems = (int)P.items.size();
   if (nitems == 0) return DGV_OK;
   if (mfma) {
@@ -429,12 +434,12 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, con
       hipLaunchKernelGGL((k_pb_mfma<DD, true>), dim3(nitems), dim3(64 * kPbWaves), 0, ctx->stream,      \
                          P.d_items,                                                                   \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_g,           \
-                         P.d_partial, g.Mv, g.Mw);                                                    \
+                         part, g.Mv, g.Mw);                                                    \
     else                                                                                              \
       hipLaunchKernelGGL((k_pb_mfma<DD, false>), dim3(nitems), dim3(64 * kPbWaves), 0, ctx->stream,     \
                          P.d_items,                                                                   \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_f,           \
-                         P.d_partial, g.Mv, g.Mw);                                                    \
+                         part, g.Mv, g.Mw);                                                    \
     break;
       DGV_MF_CASE(8)
       DGV_MF_CASE(27)
@@ -451,11 +456,11 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, con
     if (d_g)                                                                                          \
       hipLaunchKernelGGL((k_pb_eval<DD, true>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,    \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_g,           \
-                         P.d_partial, g.Mv, g.Mw);                                                    \
+                         part, g.Mv, g.Mw);                                                    \
     else                                                                                              \
       hipLaunchKernelGGL((k_pb_eval<DD, false>), dim3(nitems), dim3(64), 0, ctx->stream, P.d_items,   \
                          pb.d_groups, pb.d_group_blocks, pb.d_blocks, pb.d_pairs, d_f, d_f,           \
-                         P.d_partial, g.Mv, g.Mw);                                                    \
+                         part, g.Mv, g.Mw);                                                    \
     break;
     DGV_PB_CASE(1)
     DGV_PB_CASE(8)
@@ -468,14 +473,15 @@ static int launch_main_pairblock(dgv_ctx* ctx, const double* d_f, bool mfma, con
   return DGV_OK;
 }
 
-static int launch_reduce(dgv_ctx* ctx, double* d_q, double scale) {
+static int launch_reduce(dgv_ctx* ctx, double* d_q, double scale, const double* partial = nullptr) {
   const Grid& g = ctx->g;
   const Plan& P = ctx->plan;
+  const double* const part = partial ? partial : P.d_partial;
   switch (g.D) {
 #define DGV_RED_CASE(DD)                                                                               \
   case DD:                                                                                             \
     hipLaunchKernelGGL(k_pb_reduce<DD>, dim3((unsigned)g.ncells), dim3(64), 0, ctx->stream, g.ncells,  \
-                       P.d_cell_ptr, P.d_cell_rows, P.d_partial, ctx->d_gwts, scale, d_q);             \
+                       P.d_cell_ptr, P.d_cell_rows, part, ctx->d_gwts, scale, d_q);                    \
     break;
     DGV_RED_CASE(1)
     DGV_RED_CASE(8)
@@ -535,6 +541,31 @@ int eval_mixed_dev(dgv_ctx* ctx, const double* d_f, const double* d_fm, double* 
   int rc = launch_main_pairblock(ctx, d_f, k == DGV_KERNEL_MFMA, d_fm);
   if (rc) return rc;
   return launch_reduce(ctx, d_q, 1.0);
+}
+
+// ---- the allocation-free evaluator -------------------------------------------------
+// A file-scope device array holds the partial sums (and the projection's
+// moments) of dgv_eval_q_dev_noalloc, for callers that cannot allocate device
+// memory (code under HIP graph capture, tight serving loops).  Each GPU holds its
+// own copy from the library's load to the end of the process; its address on a
+// device comes from hipGetSymbolAddress, looked up once per device when the
+// first context is created there.  Every call overwrites what it reads, so no
+// clearing is needed; calls that overlap on one GPU (two streams, two threads)
+// share the array and race.
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) double scratch[kStaticScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int static_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return DGV_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DGV_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DGV_ERR_HIP;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return DGV_OK;
 }
 
 int eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_q, double scale) {
@@ -861,10 +892,13 @@ __global__ void k_project_apply(int64_t n, double* __restrict__ q, const double*
   q[i] = q[i] - (lam[0] + lam[1] * uu + lam[2] * vv + lam[3] * ww + lam[4] * (uu * uu + vv * vv + ww * ww));
 }
 
-int device_project(dgv_ctx* ctx, double* d_q) {
+// scratch: kRedBlocks * 5 + 16 doubles for the partial moments and the
+// multipliers (the context's reduction buffer unless given); the Cholesky
+// factor is read from the context's buffer either way
+int device_project(dgv_ctx* ctx, double* d_q, double* scratch) {
   const Grid& g = ctx->g;
-  double* part = ctx->d_red;
-  double* lam = ctx->d_red + kRedBlocks * 5 + 8;
+  double* part = scratch ? scratch : ctx->d_red;
+  double* lam = part + kRedBlocks * 5 + 8;
   double* L = ctx->d_red + kRedBlocks * 5 + 16;
   hipLaunchKernelGGL(k_moments_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, 2, g.N, d_q, ctx->d_gwts,
                      ctx->d_u, ctx->d_v, ctx->d_w, lam, part);
@@ -876,3 +910,36 @@ int device_project(dgv_ctx* ctx, double* d_q) {
 }
 
 }  // namespace dgv
+
+using namespace dgv;
+
+extern "C" int dgv_eval_q_dev_noalloc(dgv_ctx* ctx, const double* d_f, double* d_fc, int64_t n) {
+  if (!ctx) {
+    set_global_error("null context");
+    return DGV_ERR_ARG;
+  }
+  if (n != ctx->g.N) return set_error(ctx, DGV_ERR_ARG, "array length differs from the number of nodes");
+  if (!d_f || !d_fc) return set_error(ctx, DGV_ERR_ARG, "null device pointer");
+  if (!ctx->has_a) return set_error(ctx, DGV_ERR_STATE, "no A-array set (dgv_ctx_set_aarray / generate)");
+  double* const sp = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  if (!sp) return set_error(ctx, DGV_ERR_STATE, "the static scratch of this device was not set up");
+  const Grid& g = ctx->g;
+  const int k = resolve_kernel(ctx);
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  int rc;
+  size_t used = 0;
+  if (k == DGV_KERNEL_RECORD) {   // the record kernel keeps its sums on chip
+    rc = launch_eval_record(ctx, d_f, d_fc, 1.0);
+  } else {
+    used = (size_t)ctx->plan.nrows * (size_t)g.D;
+    const size_t need = (used + (ctx->project ? kRedBlocks * 5 + 16 : 0)) * sizeof(double);
+    if (need > kStaticScratchBytes)
+      return set_error(ctx, DGV_ERR_UNSUPPORTED,
+                       "partial sums of " + std::to_string(need) + " bytes exceed the static scratch of " +
+                           std::to_string(kStaticScratchBytes) + " bytes: raise the tile (dgv_ctx_set_tile)");
+    rc = launch_main_pairblock(ctx, d_f, k == DGV_KERNEL_MFMA, nullptr, sp);
+    if (!rc) rc = launch_reduce(ctx, d_fc, 1.0, sp);
+  }
+  if (!rc && ctx->project) rc = device_project(ctx, d_fc, sp + used);
+  return rc;
+}

@@ -181,6 +181,14 @@ int dgv_aarray_chunk_name(const dgv_grid_desc* grid, const char* dir, const char
  * fc(i) = (2/w_i) sum_r A_r f(xi_r + d_i) f(xi1_r + d_i) over in-grid records. */
 int dgv_eval_q(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
 int dgv_eval_q_dev(dgv_ctx* ctx, const double* d_f, double* d_fc, int64_t n);
+/* No counterpart in the reference: dgv_eval_q_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * It allocates nothing: the partial sums (and the projection's moments) go to
+ * the library's static device array of 128 MiB per GPU, held from the library's
+ * load to the end of the process.  A plan whose partial sums exceed it returns
+ * DGV_ERR_UNSUPPORTED before any launch (raise the tile).  Calls that overlap on
+ * one GPU (two streams or threads) share the array and race. */
+int dgv_eval_q_dev_noalloc(dgv_ctx* ctx, const double* d_f, double* d_fc, int64_t n);
 /* EvalCollisionPeriodicA_DGV(f,fc) (collision_mod.f90:29-129): same quantity */
 int dgv_eval_q_a(dgv_ctx* ctx, const double* f, double* fc, int64_t n);
 /* EvalCollisionPeriodicMixedTermsA_DGV(f,fm,fc) (collision_mod.f90:244-332):
