@@ -259,8 +259,9 @@ def main():
         lin = linear_leg(d, f, u, v, w, gw, rng)
 
     # ---- the batching and precision configurations ------------------------------------------
-    mixed = prec33 = None
+    mixed = prec33 = plain = None
     if not args.no_configs:
+        plain = plain_config_leg(dgv, args)
         mixed = mixed_grid_leg(dgv, d, rng, args)
         prec33 = precision_leg(dgv, rng, args)
 
@@ -344,6 +345,7 @@ def main():
         "rk3": rk,
         "surrogate": sur,
         "linearised_regime": lin,
+        "plain_15cubed": plain,
         "mixed_grid_batch": mixed,
         "precision_33cubed": prec33,
     }
@@ -433,6 +435,41 @@ def cpu_baseline_leg(d, f, q_single, M, L, args):
                       f"{nproc} host threads, scaled to N nodes (extrapolated); 1 core: {len(n1)} nodes in "
                       f"{cs1:.1f} s; GPU FP64 vs oracle on the sample rel L2 {err:.2e}. 15^3 plain configuration: one "
                       f"full Q in {c15:.2f} s on {threads} threads (config = 201 Q)"}
+
+
+def plain_config_leg(dgv, args):
+    """The plain configuration on the GPU: 15^3 nodes (5 cells on [-3.5,3.5]^3),
+    the seeded bimodal VDF, one Q(f) evaluation plus 100 RK2 steps at dt = 0.01
+    (201 Q evaluations), FP64, timed end to end with the steps as hipGraph
+    replays; mass drift reported."""
+    from dgv0d3v_direct_collision_amd.workloads import bimodal_vdf
+    g = dgv.DGV(5, 3, (-3.5, 3.5))
+    t0 = time.perf_counter()
+    g.generate_aarray(min_sens=0.0, nquad=args.nquad)
+    g.synchronize()
+    agen = time.perf_counter() - t0
+    f = bimodal_vdf(g.nodes_u, g.nodes_v, g.nodes_w, np.random.default_rng(20261202))
+    d_f = g.device_alloc(8 * g.nnodes)
+    d_q = g.device_alloc(8 * g.nnodes)
+    g.htod(d_f, f)
+    g.eval_q_dev(d_f, d_q)
+    g.rk_run_dev(d_f, 0.01, 1, rkmts=2, use_graph=True)      # capture
+    g.htod(d_f, f)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.eval_q_dev(d_f, d_q)
+    g.rk_run_dev(d_f, 0.01, 100, rkmts=2, use_graph=True)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    fr = np.zeros(g.nnodes)
+    g.dtoh(fr, d_f)
+    drift = float(np.sum(g.nodes_gwts * fr) - np.sum(g.nodes_gwts * f))
+    for p in (d_f, d_q):
+        g.device_free(p)
+    g.close()
+    log(f"plain 15^3: 1 Q + 100 RK2 steps in {el * 1e3:.1f} ms ({201 / el:.0f} Q/s); density drift {drift:.3e}")
+    return {"nodes": 3375, "q_evals": 201, "seconds": el, "q_per_s": 201 / el, "rk2_steps_per_s": 100 / el,
+            "agen_seconds": agen, "density_drift": drift}
 
 
 def mixed_grid_leg(dgv, d27, rng, args):

@@ -136,6 +136,7 @@ namespace dgv {
 static int berr(dgv_batch* b, int code, const std::string& m) {
   if (b) b->err = m;
   set_global_error(m);
+  if (code == DGV_ERR_NOMEM) (void)hipGetLastError();   // the failed hipMalloc's sticky error
   return code;
 }
 
@@ -658,6 +659,12 @@ __global__ void k_i8_fixed(int64_t ncells, int32_t Bpad, const double* __restric
 // instances per item, two waves per chunk (32 instances each: column tiles
 // ct = 0, 1, column jj of tile ct = instance 2 jj + ct of the wave's 32), as the
 // FP64 mode.  A 16-pair step: per row tile 5 level MFMAs per column tile.
+// The digit and code operands come through per-block buffer resources from a
+// ring of kI8Ring steps in flight (every array ends with kI8Ring zero steps, so
+// the ring reads straight ahead); the LDS reads of a step's f values are
+// issued one step ahead of its products.
+constexpr int kI8Ring = 8;
+
 template <int D>
 __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ items, int nitems,
                                                   const BatchGridDev* __restrict__ grids,
@@ -666,6 +673,7 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
   constexpr int RT = (D + 15) / 16;
   constexpr int CH = 2, WPC = 2, CT = 2;
   constexpr int DW = i8_dwords_per_pair<D>();     // digit dwords per pair
+  constexpr uint32_t kStepA = 16u * DW * 4u;      // digit bytes per 16-pair step
   typedef double VB __attribute__((ext_vector_type(2)));
   __shared__ double lfa[CH][D * 64], lfb[CH][D * 64];
   __shared__ int s_item;
@@ -674,6 +682,8 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
   const int ch = wave / WPC, sub = wave % WPC;
   const int col0 = sub * 32;
   const int lcol = col0 + 2 * jj;                  // the lane's instances in a staged row: lcol, lcol + 1
+  const uint32_t aoff = (uint32_t)lane * 32u;      // the lane's 8 digit dwords of a step's row tile
+  const uint32_t coff = (uint32_t)kk * 16u;        // its lane group's 4 codes of a step
   const bi4 izero = bi4{0, 0, 0, 0};
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -704,17 +714,17 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
     double* lb = lfb[ch];
     int prev_xc = -1;
     for (int bi = it.b0; bi < it.b1; ++bi) {
-      const int blk = G.blist[bi];
+      const int blk = __builtin_amdgcn_readfirstlane(G.blist[bi]);
       const BlockHdr* B = G.blocks + blk;
       const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
       const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
       const bool new_x = xc != prev_xc;
       prev_xc = xc;
-      const int n = B->npairs;
+      const int n = __builtin_amdgcn_readfirstlane((int)B->npairs);
       const int nst = (n + 15) >> 4;
       const int64_t o16 = G.pbeg16[blk];
-      const bi4* Ablk = reinterpret_cast<const bi4*>(G.digits + o16 * DW) + lane * 2;
-      const bu4* Cblk = reinterpret_cast<const bu4*>(G.codes + o16) + kk;
+      const __amdgpu_buffer_rsrc_t RA = ring_rsrc(G.digits + o16 * DW);
+      const __amdgpu_buffer_rsrc_t RC = ring_rsrc(G.codes + o16);
       // scales of the block: the lane's output rows (4 kk + e of tile rt) and columns
       double sA[RT][4], sF[CT] = {};
 #pragma unroll
@@ -729,16 +739,22 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
           sF[ct] = ldexp(1.0, ea + eb - 2 * kI8FS);
         }
       }
-      // first step's operands ahead of the staging
-      bi4 a1[RT], a2[RT];
-      bu4 cd;
+      // the ring, filled ahead of the staging so that both latencies overlap
+      bi4 r1[kI8Ring][RT], r2[kI8Ring][RT];
+      bu4 rc[kI8Ring];
+#define I8_LOAD(slot, step)                                                                         \
+  do {                                                                                              \
+    const uint32_t sa_ = kStepA * (uint32_t)(step);                                                 \
+    _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                             \
+      r1[slot][rt] = __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA, aoff, sa_ + rt * 2048u, 0)); \
+      r2[slot][rt] =                                                                                \
+          __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA, aoff + 16u, sa_ + rt * 2048u, 0)); \
+    }                                                                                               \
+    rc[slot] = __builtin_amdgcn_raw_buffer_load_b128(RC, coff, 64u * (uint32_t)(step), 0);          \
+  } while (0)
       if (active && n > 0) {
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          a1[rt] = Ablk[rt * 128];
-          a2[rt] = Ablk[rt * 128 + 1];
-        }
-        cd = Cblk[0];
+        for (int u = 0; u < kI8Ring; ++u) I8_LOAD(u, u);
       }
       __syncthreads();
       if (active) {   // waves 2c and 2c+1 split the rows of chunk c
@@ -764,55 +780,54 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       if (!active || n == 0) continue;
       const char* la_lane = reinterpret_cast<const char*>(la + lcol);
       const char* lb_lane = reinterpret_cast<const char*>(lb + lcol);
-      for (int st = 0; st < nst; ++st) {
-        // the x dwords of the lane's four pairs, for both column tiles
-        bi4 nat[CT];
+      VB fa[4], fb[4];   // the f values of the step whose products come next
+#define I8_LDS(cd_)                                                                                 \
+  do {                                                                                              \
+    _Pragma("unroll") for (int m = 0; m < 4; ++m) {                                                 \
+      fa[m] = *reinterpret_cast<const VB*>(la_lane + ((cd_)[m] & 0xffffu));                         \
+      fb[m] = *reinterpret_cast<const VB*>(lb_lane + ((cd_)[m] >> 16));                             \
+    }                                                                                               \
+  } while (0)
+      // one step: the x dwords of the lane's four pairs for both column tiles,
+      // the next step's LDS reads, then the 5 x RT x CT products
+#define I8_STEP(slot, next_slot)                                                                    \
+  do {                                                                                              \
+    bi4 nat_[CT];                                                                                   \
+    _Pragma("unroll") for (int m = 0; m < 4; ++m)                                                   \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                               \
+      nat_[ct][m] = (int)(__double2uint_rz(fma(fa[m][ct], fb[m][ct], kI8Bias)) ^ 0x80808080u);       \
+    I8_LDS(rc[next_slot]);                                                                          \
+    __builtin_amdgcn_sched_barrier(0);                                                              \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                             \
+      const bu4 u3_ = __builtin_bit_cast(bu4, nat_[ct]);                                            \
+      const bi4 x2_ = __builtin_bit_cast(bi4, u3_ >> 8u);                                           \
+      const bi4 x1_ = __builtin_bit_cast(bi4, u3_ >> 16u);                                          \
+      const bi4 x0_ = __builtin_bit_cast(bi4, u3_ >> 24u);                                          \
+      _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                           \
+        acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], nat_[ct], acc[3][rt][ct], 0, 0, 0); \
+        acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r2[slot][rt], nat_[ct], acc[4][rt][ct], 0, 0, 0); \
+        acc[2][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x2_, acc[2][rt][ct], 0, 0, 0); \
+        acc[1][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x1_, acc[1][rt][ct], 0, 0, 0); \
+        acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x0_, acc[0][rt][ct], 0, 0, 0); \
+      }                                                                                             \
+    }                                                                                               \
+    __builtin_amdgcn_sched_barrier(0);                                                              \
+  } while (0)
+      I8_LDS(rc[0]);
+      int st = 0;
+      for (; st + kI8Ring <= nst; st += kI8Ring) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const uint32_t code = cd[m];
-          const VB ra = *reinterpret_cast<const VB*>(la_lane + (code & 0xffffu));
-          const VB rb = *reinterpret_cast<const VB*>(lb_lane + (code >> 16));
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) {
-            const double y = fma(ra[ct], rb[ct], kI8Bias);
-            nat[ct][m] = (int)(__double2uint_rz(y) ^ 0x80808080u);
-          }
-        }
-        // the next step's operands
-        bi4 n1[RT], n2[RT];
-        bu4 ncd = cd;
-        if (st + 1 < nst) {
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt) {
-            n1[rt] = Ablk[((st + 1) * RT + rt) * 128];
-            n2[rt] = Ablk[((st + 1) * RT + rt) * 128 + 1];
-          }
-          ncd = Cblk[(st + 1) * 4];
-        }
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          const bu4 u3 = __builtin_bit_cast(bu4, nat[ct]);
-          const bi4 x2 = __builtin_bit_cast(bi4, u3 >> 8u);
-          const bi4 x1 = __builtin_bit_cast(bi4, u3 >> 16u);
-          const bi4 x0 = __builtin_bit_cast(bi4, u3 >> 24u);
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt) {
-            acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x0, acc[0][rt][ct], 0, 0, 0);
-            acc[1][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x1, acc[1][rt][ct], 0, 0, 0);
-            acc[2][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], x2, acc[2][rt][ct], 0, 0, 0);
-            acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[rt], nat[ct], acc[3][rt][ct], 0, 0, 0);
-            acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a2[rt], nat[ct], acc[4][rt][ct], 0, 0, 0);
-          }
-        }
-        if (st + 1 < nst) {
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt) {
-            a1[rt] = n1[rt];
-            a2[rt] = n2[rt];
-          }
-          cd = ncd;
+        for (int u = 0; u < kI8Ring; ++u) {
+          I8_STEP(u, (u + 1) % kI8Ring);
+          I8_LOAD(u, st + u + kI8Ring);
         }
       }
+#pragma unroll
+      for (int u = 0; u < kI8Ring; ++u)   // the last nst - st < kI8Ring steps, already in the ring
+        if (st + u < nst) I8_STEP(u, (u + 1) % kI8Ring);
+#undef I8_STEP
+#undef I8_LDS
+#undef I8_LOAD
       // the block's exact sums, combined and scaled into the FP64 running sums
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
@@ -1272,7 +1287,12 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     off += grids[g]->g.N * (int64_t)b->Bpad[g];
   }
   b->total = off;
-  auto fail = [&](int code, const std::string& m) { dgv_batch_destroy(b); set_global_error(m); return code; };
+  auto fail = [&](int code, const std::string& m) {
+    dgv_batch_destroy(b);
+    set_global_error(m);
+    if (code == DGV_ERR_NOMEM) (void)hipGetLastError();   // the failed hipMalloc's sticky error
+    return code;
+  };
   if (hipSetDevice(grids[0]->device) != hipSuccess) return fail(DGV_ERR_HIP, "hipSetDevice failed");
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, grids[0]->device) == hipSuccess) b->ncu = prop.multiProcessorCount;
@@ -1352,13 +1372,17 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
       const int RT = (D + 15) / 16;
       const int64_t dw = (int64_t)RT * 32;   // digit dwords per pair
       auto& a = b->d_i8[g];
-      if (hipMalloc(&a[0], sizeof(uint32_t) * std::max<int64_t>(tot16 * dw, 1)) ||
-          hipMalloc(&a[1], sizeof(uint32_t) * std::max<int64_t>(tot16, 1)) ||
+      const int64_t pad16 = 16 * kI8Ring;    // the ring reads up to kI8Ring steps past a block
+      if (hipMalloc(&a[0], sizeof(uint32_t) * (tot16 + pad16) * dw) ||
+          hipMalloc(&a[1], sizeof(uint32_t) * (tot16 + pad16)) ||
           hipMalloc(&a[2], sizeof(double) * 16 * RT * std::max<size_t>(pb.blocks.size(), 1)) ||
           hipMalloc(&a[3], sizeof(int64_t) * std::max<size_t>(p16.size(), 1)))
         return fail(DGV_ERR_NOMEM, "hipMalloc(INT8_SPLIT coefficient digits)");
       if (hipMemcpy(a[3], p16.data(), sizeof(int64_t) * p16.size(), hipMemcpyHostToDevice))
         return fail(DGV_ERR_HIP, "hipMemcpy(block starts)");
+      if (hipMemsetAsync((uint32_t*)a[0] + tot16 * dw, 0, sizeof(uint32_t) * pad16 * dw, b->stream) ||
+          hipMemsetAsync((uint32_t*)a[1] + tot16, 0, sizeof(uint32_t) * pad16, b->stream))
+        return fail(DGV_ERR_HIP, "hipMemsetAsync(digit padding)");
       if (!pb.blocks.empty()) {
         if (D == 27)
           hipLaunchKernelGGL(k_i8_digits<27>, dim3((unsigned)pb.blocks.size()), dim3(256), 0, b->stream, pb.d_blocks,
