@@ -549,9 +549,10 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
 // c the f of every (cell, instance) is scaled by a power of two so that its
 // largest value lies in [2^14, 2^15): fh = f 2^(15 - e).  A pair's product
 // x = fh(c+a, p) fh(c+b, q), |x| < 2^30, is rounded to an integer and written as
-// four signed digits X_0 .. X_3 in the bytes of one dword (X_0 in the top byte),
-// by rounding x + 0x80808080 to an unsigned integer and flipping each byte's top
-// bit.  Then At x = sum_k 256^(7-k) sum_{t+s=k} A_t X_s, and the levels k = 0..4
+// four signed digits X_0 .. X_3 in the bytes of one dword (X_0 in the top byte):
+// x + 0x80808080 is rounded to an integer by adding it to 2^52 in FP64 (the low
+// 32 bits of the sum's significand are then that integer), and each byte's top
+// bit is flipped.  Then At x = sum_k 256^(7-k) sum_{t+s=k} A_t X_s, and the levels k = 0..4
 // (all digit products down to 256^-4 of the leading one) are accumulated exactly
 // in int32 by v_mfma_i32_16x16x64_i8:
 //   level 3: A dword (A_0, A_1, A_2, A_3) against the natural x dword;
@@ -567,7 +568,7 @@ typedef unsigned int bu4 __attribute__((ext_vector_type(4)));
 constexpr int kI8BA = 38;                   // coefficient bits: |At| < 2^38
 constexpr int kI8BX = 30;                   // product bits: |x| < 2^30
 constexpr int kI8FS = 15;                   // f scaled into [2^14, 2^15)
-constexpr double kI8Bias = 2155905152.5;    // 0x80808080 + 1/2 (rounds to nearest)
+constexpr double kI8Magic = 4503599627370496.0 + 2155905152.0;   // 2^52 + 0x80808080
 
 template <int D>
 constexpr int i8_dwords_per_pair() { return ((D + 15) / 16) * 16 * 2; }   // 2 dwords per tile row
@@ -631,7 +632,10 @@ __global__ __launch_bounds__(256) void k_i8_digits(const BlockHdr* __restrict__ 
   }
   for (int j = threadIdx.x; j < n16; j += blockDim.x) {
     uint32_t c = 0;   // padded pairs: rows 0 and 0, zero digits
-    if (j < n) c = (uint32_t)((uint64_t)__double_as_longlong(pairs64[(p0 + j) * (D + 1) + D]) >> 32);
+    if (j < n) {   // LDS byte offsets of the pair's rows p, q in a 32-instance staging buffer
+      const uint32_t pq = (uint32_t)__double_as_longlong(pairs64[(p0 + j) * (D + 1) + D]) & 0xffffu;
+      c = ((pq >> 8) << 8) | ((pq & 0xffu) << 24);
+    }
     codes[o16 + j] = c;
   }
 }
@@ -655,15 +659,46 @@ __global__ void k_i8_fixed(int64_t ncells, int32_t Bpad, const double* __restric
   }
 }
 
-// The INT8_SPLIT evaluator.  Workgroup of four waves per item, two chunks of 64
-// instances per item, two waves per chunk (32 instances each: column tiles
-// ct = 0, 1, column jj of tile ct = instance 2 jj + ct of the wave's 32), as the
-// FP64 mode.  A 16-pair step: per row tile 5 level MFMAs per column tile.
-// The digit and code operands come through per-block buffer resources from a
-// ring of kI8Ring steps in flight (every array ends with kI8Ring zero steps, so
-// the ring reads straight ahead); the LDS reads of a step's f values are
-// issued one step ahead of its products.
+// The INT8_SPLIT evaluator.  Workgroup of four waves per item: two chunks of 64
+// instances per item, two waves per chunk, 32 instances per wave (column tiles
+// ct = 0, 1; column jj of tile ct = instance 2 jj + ct of the wave's 32).  A
+// 16-pair step: per row tile 5 level MFMAs per column tile.
+// Inside an item the waves run independently, and the block boundaries are
+// hidden behind the products:
+//  - the digit and code operands come through a ring of kI8Ring steps in
+//    flight (per-block buffer resources).  Once a slot's last step of a block
+//    has been used, the slot loads the same-numbered step of the next block, so
+//    the next block's first steps are in flight before the current one ends
+//    (every digit array ends with kI8Ring zero steps: the ring reads ahead);
+//  - each wave stages its own 32 instances of f(c+a), f(c+b) and the block's
+//    scales in LDS with direct global-to-LDS loads, double-buffered: the next
+//    block's staging is issued when a block starts and has landed when the next
+//    block starts, since at least a full ring of loads was issued after it and
+//    the wave waits for all but that many;
+//  - the block headers come through the scalar cache, two blocks ahead;
+//  - the LDS reads of a step's f values are issued two steps ahead and its x
+//    dwords computed one step ahead of its products.
 constexpr int kI8Ring = 8;
+
+// s_waitcnt immediates: vector memory count n (lgkm and exp counts left free),
+// and LDS/scalar count 0 (vector memory and exp left free)
+__device__ __forceinline__ constexpr int i8_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+constexpr int kI8LgkmZero = 15 | (7 << 4) | (3 << 14);
+
+// Direct global-to-LDS loads: lane l's 16 (4) bytes from g land at LDS byte
+// address lds + 16 l (4 l).  Written out so that the compiler does not hold the
+// wave's LDS reads of the other buffers behind them; the kernel orders them
+// itself (vector-memory count waits before the staged data is read).
+__device__ __forceinline__ void i8_lds_load16(const void* g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(g) : "memory", "m0");
+}
+__device__ __forceinline__ void i8_lds_load4(const void* g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"(lds), "v"(g) : "memory", "m0");
+}
+template <typename T>
+__device__ __forceinline__ uint32_t lds_addr(T* p) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) T*)p));
+}
 
 template <int D>
 __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ items, int nitems,
@@ -671,17 +706,24 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
                                                   const double* __restrict__ fhat, const int32_t* __restrict__ fexp,
                                                   double* __restrict__ partial, int* __restrict__ counter) {
   constexpr int RT = (D + 15) / 16;
-  constexpr int CH = 2, WPC = 2, CT = 2;
+  constexpr int ROWS = (D + 3) & ~3;              // staged rows: four per 1 KiB direct load
+  constexpr int CT = 2, WPC = 2;
   constexpr int DW = i8_dwords_per_pair<D>();     // digit dwords per pair
   constexpr uint32_t kStepA = 16u * DW * 4u;      // digit bytes per 16-pair step
+  constexpr int kRowBytes = 32 * 8;               // a staged row: the wave's 32 instances
+  constexpr int kBufBytes = ROWS * kRowBytes;
+  constexpr int kSlotLoads = 2 * RT + 1;          // buffer loads per ring slot
   typedef double VB __attribute__((ext_vector_type(2)));
-  __shared__ double lfa[CH][D * 64], lfb[CH][D * 64];
+  typedef const __attribute__((address_space(4))) int32_t* ScalarI32;
+  typedef const __attribute__((address_space(4))) int64_t* ScalarI64;
+  __shared__ __attribute__((aligned(16))) char s_f[4][2][2][kBufBytes];   // [wave][f(c+a) | f(c+b)][buffer]
+  __shared__ __attribute__((aligned(16))) double s_sa[4][2][32];         // [wave][block parity]: row scales
+  __shared__ __attribute__((aligned(16))) int s_fe[4][2][64];            // [wave][parity]: exponents of f(c+a), f(c+b)
   __shared__ int s_item;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kk = lane >> 4, jj = lane & 15;
   const int ch = wave / WPC, sub = wave % WPC;
-  const int col0 = sub * 32;
-  const int lcol = col0 + 2 * jj;                  // the lane's instances in a staged row: lcol, lcol + 1
+  const int lcol = sub * 32 + 2 * jj;              // the lane's instances in the chunk: lcol, lcol + 1
   const uint32_t aoff = (uint32_t)lane * 32u;      // the lane's 8 digit dwords of a step's row tile
   const uint32_t coff = (uint32_t)kk * 16u;        // its lane group's 4 codes of a step
   const bi4 izero = bi4{0, 0, 0, 0};
@@ -692,95 +734,99 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
     __syncthreads();
     if (id >= nitems) return;
     const BatchItem it = items[id];
+    if (ch >= it.nvc) continue;   // an idle chunk: its columns lie past the grid's instances
     const BatchGridDev G = grids[it.grid];
-    const bool active = ch < it.nvc;
     const int Mv = G.Mv, Mw = G.Mw;
     const int c = it.cell;
     const int cw = c % Mw, cv = (c / Mw) % Mv, cu = c / (Mw * Mv);
-    const int64_t vb = (int64_t)(it.vc0 + ch) * 64 + lane;          // the instance this thread stages
-    const int64_t vcol = (int64_t)(it.vc0 + ch) * 64 + lcol;        // the lane's first column's instance
+    const int64_t Bpad = G.Bpad;
+    const int64_t vcol0 = (int64_t)(it.vc0 + ch) * 64 + sub * 32;   // the wave's first instance
+    const double* F = fhat + G.foff;
+    const int32_t* FE = fexp + G.eoff;
     double tot[RT][CT][4];
-    bi4 acc[5][RT][CT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int e = 0; e < 4; ++e) tot[rt][ct][e] = 0.0;
+    const int nb = it.b1 - it.b0;
+    if (nb > 0) {
+      const ScalarI32 blist = (ScalarI32)(G.blist + it.b0);
+      // a block's header: its records, step count and the cells of f(c+a), f(c+b)
+      struct Hdr {
+        int blk, nst, xc, yc;
+        const uint32_t* A;
+        const uint32_t* C;
+      };
+      auto header = [&](int blk) {
+        Hdr h;
+        h.blk = blk;
+        const ScalarI32 w = (ScalarI32)(reinterpret_cast<const char*>(G.blocks + blk) + 8);
+        const int w0 = w[0], w1 = w[1];   // a[0..2], b[0]; b[1..2], npairs
+        const int a0 = (int8_t)w0, a1 = (int8_t)(w0 >> 8), a2 = (int8_t)(w0 >> 16);
+        const int b0 = (int8_t)(w0 >> 24), b1 = (int8_t)w1, b2 = (int8_t)(w1 >> 8);
+        h.nst = ((int)(int16_t)(w1 >> 16) + 15) >> 4;
+        h.xc = ((cu + a0) * Mv + (cv + a1)) * Mw + (cw + a2);
+        h.yc = ((cu + b0) * Mv + (cv + b1)) * Mw + (cw + b2);
+        const int64_t o16 = ((ScalarI64)G.pbeg16)[blk];
+        h.A = G.digits + o16 * DW;
+        h.C = G.codes + o16;
+        return h;
+      };
+      // direct loads of f rows of one cell (the wave's 32 instances) into a buffer
+      auto stage_rows = [&](int cell, char* buf) {
+        const double* src = F + (int64_t)cell * D * Bpad + vcol0 + 2 * jj;
+        const uint32_t lb = lds_addr(buf);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) acc[k][rt][ct] = izero;
-      }
-    double* la = lfa[ch];
-    double* lb = lfb[ch];
-    int prev_xc = -1;
-    for (int bi = it.b0; bi < it.b1; ++bi) {
-      const int blk = __builtin_amdgcn_readfirstlane(G.blist[bi]);
-      const BlockHdr* B = G.blocks + blk;
-      const int xc = ((cu + B->a[0]) * Mv + (cv + B->a[1])) * Mw + (cw + B->a[2]);
-      const int yc = ((cu + B->b[0]) * Mv + (cv + B->b[1])) * Mw + (cw + B->b[2]);
-      const bool new_x = xc != prev_xc;
-      prev_xc = xc;
-      const int n = __builtin_amdgcn_readfirstlane((int)B->npairs);
-      const int nst = (n + 15) >> 4;
-      const int64_t o16 = G.pbeg16[blk];
-      const __amdgpu_buffer_rsrc_t RA = ring_rsrc(G.digits + o16 * DW);
-      const __amdgpu_buffer_rsrc_t RC = ring_rsrc(G.codes + o16);
-      // scales of the block: the lane's output rows (4 kk + e of tile rt) and columns
-      double sA[RT][4], sF[CT] = {};
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) sA[rt][e] = G.bscale[(int64_t)blk * 16 * RT + 16 * rt + 4 * kk + e];
-      if (active) {   // an idle chunk's columns lie past the grid's instances
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          const int ea = fexp[G.eoff + (int64_t)xc * G.Bpad + vcol + ct];
-          const int eb = fexp[G.eoff + (int64_t)yc * G.Bpad + vcol + ct];
-          sF[ct] = ldexp(1.0, ea + eb - 2 * kI8FS);
+        for (int i = 0; i < ROWS / 4; ++i) {
+          const int p = min(4 * i + kk, D - 1);
+          i8_lds_load16(src + p * Bpad, lb + i * 1024);
         }
-      }
-      // the ring, filled ahead of the staging so that both latencies overlap
+      };
+      // ... and of a block's row scales and the exponents of its two cells
+      auto stage_scales = [&](const Hdr& h, int par) {
+        const char* sa = reinterpret_cast<const char*>(G.bscale + (int64_t)h.blk * 16 * RT) + ((lane * 4) & (16 * RT * 8 - 1));
+        i8_lds_load4(sa, lds_addr(&s_sa[wave][par][0]));
+        const int32_t* se = FE + (int64_t)(lane < 32 ? h.xc : h.yc) * Bpad + vcol0 + (lane & 31);
+        i8_lds_load4(se, lds_addr(&s_fe[wave][par][0]));
+      };
+      // the cell held by each f buffer
+      int tag_a0 = -1, tag_a1 = -1, tag_b0 = -1, tag_b1 = -1;
+      Hdr cur = header(blist[0]);
+      Hdr nxt = header(blist[min(1, nb - 1)]);
+      int bq = blist[min(2, nb - 1)];   // the block after nxt
+      int pa = 0, pb = 0;               // the current block's f buffers
+      stage_rows(cur.xc, s_f[wave][0][0]);
+      stage_rows(cur.yc, s_f[wave][1][0]);
+      tag_a0 = cur.xc;
+      tag_b0 = cur.yc;
+      stage_scales(cur, 0);
       bi4 r1[kI8Ring][RT], r2[kI8Ring][RT];
       bu4 rc[kI8Ring];
-#define I8_LOAD(slot, step)                                                                         \
+#define I8_LOAD(slot, pA, pC, step)                                                                 \
   do {                                                                                              \
+    const __amdgpu_buffer_rsrc_t RA_ = ring_rsrc(pA);                                               \
+    const __amdgpu_buffer_rsrc_t RC_ = ring_rsrc(pC);                                               \
     const uint32_t sa_ = kStepA * (uint32_t)(step);                                                 \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                             \
-      r1[slot][rt] = __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA, aoff, sa_ + rt * 2048u, 0)); \
+      r1[slot][rt] = __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA_, aoff, sa_ + rt * 2048u, 0)); \
       r2[slot][rt] =                                                                                \
-          __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA, aoff + 16u, sa_ + rt * 2048u, 0)); \
+          __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA_, aoff + 16u, sa_ + rt * 2048u, 0)); \
     }                                                                                               \
-    rc[slot] = __builtin_amdgcn_raw_buffer_load_b128(RC, coff, 64u * (uint32_t)(step), 0);          \
+    rc[slot] = __builtin_amdgcn_raw_buffer_load_b128(RC_, coff, 64u * (uint32_t)(step), 0);         \
   } while (0)
-      if (active && n > 0) {
+      // in slot order, as in the loop (the loop's waits count loads in that order)
 #pragma unroll
-        for (int u = 0; u < kI8Ring; ++u) I8_LOAD(u, u);
+      for (int u = 0; u < kI8Ring; ++u) {
+        I8_LOAD(u, cur.A, cur.C, u);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
-      if (active) {   // waves 2c and 2c+1 split the rows of chunk c
-        const double* F = fhat + G.foff;
-        constexpr int NS = (D + WPC - 1) / WPC;
-        double sa[NS], sb[NS];
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-          const int p = min(sub + i * WPC, D - 1);
-          if (new_x) sa[i] = F[((int64_t)xc * D + p) * G.Bpad + vb];
-          sb[i] = F[((int64_t)yc * D + p) * G.Bpad + vb];
-        }
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-          const int p = sub + i * WPC;
-          if (p < D) {
-            if (new_x) la[p * 64 + lane] = sa[i];
-            lb[p * 64 + lane] = sb[i];
-          }
-        }
-      }
-      __syncthreads();
-      if (!active || n == 0) continue;
-      const char* la_lane = reinterpret_cast<const char*>(la + lcol);
-      const char* lb_lane = reinterpret_cast<const char*>(lb + lcol);
-      VB fa[4], fb[4];   // the f values of the step whose products come next
+      bi4 acc[5][RT][CT];
+      VB fa[4], fb[4];   // f values read from LDS two steps ahead of their products
+      bi4 nat[CT];       // the x dwords of the step whose products come next
+      const char* la_lane;
+      const char* lb_lane;
 #define I8_LDS(cd_)                                                                                 \
   do {                                                                                              \
     _Pragma("unroll") for (int m = 0; m < 4; ++m) {                                                 \
@@ -788,71 +834,134 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       fb[m] = *reinterpret_cast<const VB*>(lb_lane + ((cd_)[m] >> 16));                             \
     }                                                                                               \
   } while (0)
-      // one step: the x dwords of the lane's four pairs for both column tiles,
-      // the next step's LDS reads, then the 5 x RT x CT products
-#define I8_STEP(slot, next_slot)                                                                    \
+      // the x dwords of the lane's four pairs, for both column tiles, from fa / fb
+#define I8_NAT(out_)                                                                                \
   do {                                                                                              \
-    bi4 nat_[CT];                                                                                   \
     _Pragma("unroll") for (int m = 0; m < 4; ++m)                                                   \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct)                                               \
-      nat_[ct][m] = (int)(__double2uint_rz(fma(fa[m][ct], fb[m][ct], kI8Bias)) ^ 0x80808080u);       \
-    I8_LDS(rc[next_slot]);                                                                          \
-    __builtin_amdgcn_sched_barrier(0);                                                              \
+      (out_)[ct][m] = (int)((uint32_t)__double_as_longlong(fma(fa[m][ct], fb[m][ct], kI8Magic)) ^ 0x80808080u); \
+  } while (0)
+      // one step, software-pipelined over three steps: the next step's x dwords
+      // (VALU) and the LDS reads of the step after it are interleaved with this
+      // step's 5 x RT x CT products, so that the VALU work runs beside the
+      // matrix cores
+#define I8_STEP(slot, next2_slot)                                                                   \
+  do {                                                                                              \
+    bi4 natn_[CT];                                                                                  \
+    I8_NAT(natn_);                                                                                  \
+    I8_LDS(rc[next2_slot]);                                                                         \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                             \
-      const bu4 u3_ = __builtin_bit_cast(bu4, nat_[ct]);                                            \
+      const bu4 u3_ = __builtin_bit_cast(bu4, nat[ct]);                                             \
       const bi4 x2_ = __builtin_bit_cast(bi4, u3_ >> 8u);                                           \
       const bi4 x1_ = __builtin_bit_cast(bi4, u3_ >> 16u);                                          \
       const bi4 x0_ = __builtin_bit_cast(bi4, u3_ >> 24u);                                          \
       _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                           \
-        acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], nat_[ct], acc[3][rt][ct], 0, 0, 0); \
-        acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r2[slot][rt], nat_[ct], acc[4][rt][ct], 0, 0, 0); \
+        acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], nat[ct], acc[3][rt][ct], 0, 0, 0); \
+        acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r2[slot][rt], nat[ct], acc[4][rt][ct], 0, 0, 0); \
         acc[2][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x2_, acc[2][rt][ct], 0, 0, 0); \
         acc[1][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x1_, acc[1][rt][ct], 0, 0, 0); \
         acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x0_, acc[0][rt][ct], 0, 0, 0); \
       }                                                                                             \
     }                                                                                               \
-    __builtin_amdgcn_sched_barrier(0);                                                              \
+    /* interleave: the DS reads first, then one product per three VALU instructions */             \
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);                                              \
+    _Pragma("unroll") for (int g_ = 0; g_ < 5 * RT * CT; ++g_) {                                    \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                            \
+      __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);                                            \
+    }                                                                                               \
+    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) nat[ct] = natn_[ct];                          \
   } while (0)
-      I8_LDS(rc[0]);
-      int st = 0;
-      for (; st + kI8Ring <= nst; st += kI8Ring) {
-#pragma unroll
-        for (int u = 0; u < kI8Ring; ++u) {
-          I8_STEP(u, (u + 1) % kI8Ring);
-          I8_LOAD(u, st + u + kI8Ring);
+      for (int j = 0; j < nb; ++j) {
+        // this block's staging has landed (a full ring of loads was issued after
+        // it), and the previous block's LDS reads are done, so its buffers are free
+        __builtin_amdgcn_s_waitcnt(i8_vmcnt(kSlotLoads * kI8Ring));
+        __builtin_amdgcn_s_waitcnt(kI8LgkmZero);
+        int pa_n = pa, pb_n = pb;
+        if (j + 1 < nb) {
+          if (nxt.xc != (pa ? tag_a1 : tag_a0)) {
+            pa_n = (nxt.xc == (pa ? tag_a0 : tag_a1)) ? 1 - pa : -1;
+            if (pa_n < 0) {
+              pa_n = 1 - pa;
+              stage_rows(nxt.xc, s_f[wave][0][pa_n]);
+              if (pa_n) tag_a1 = nxt.xc; else tag_a0 = nxt.xc;
+            }
+          }
+          if (nxt.yc != (pb ? tag_b1 : tag_b0)) {
+            pb_n = (nxt.yc == (pb ? tag_b0 : tag_b1)) ? 1 - pb : -1;
+            if (pb_n < 0) {
+              pb_n = 1 - pb;
+              stage_rows(nxt.yc, s_f[wave][1][pb_n]);
+              if (pb_n) tag_b1 = nxt.yc; else tag_b0 = nxt.yc;
+            }
+          }
+          stage_scales(nxt, (j + 1) & 1);
         }
-      }
+        const Hdr nn = header(bq);
+        bq = blist[min(j + 3, nb - 1)];
 #pragma unroll
-      for (int u = 0; u < kI8Ring; ++u)   // the last nst - st < kI8Ring steps, already in the ring
-        if (st + u < nst) I8_STEP(u, (u + 1) % kI8Ring);
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k][rt][ct] = izero;
+        la_lane = s_f[wave][0][pa] + jj * 16;
+        lb_lane = s_f[wave][1][pb] + jj * 16;
+        const int nst = max(cur.nst, 1);
+        I8_LDS(rc[0]);
+        I8_NAT(nat);
+        I8_LDS(rc[1 % kI8Ring]);
+        // every slot loads once per pass (the count of loads in flight is the
+        // same on every path); past the block's end a slot loads the next
+        // block's step of its own number (again, on the last partial pass)
+        for (int st = 0; st < nst; st += kI8Ring) {
+#pragma unroll
+          for (int u = 0; u < kI8Ring; ++u) {
+            if (st + u < nst) I8_STEP(u, (u + 2) % kI8Ring);
+            const int s_ = st + u + kI8Ring;
+            const bool in_ = s_ < nst;
+            I8_LOAD(u, in_ ? cur.A : nxt.A, in_ ? cur.C : nxt.C, in_ ? s_ : u);
+          }
+        }
+        // the block's exact sums, combined and scaled into the FP64 running sums
+        {
+          const int par = j & 1;
+          double sA[RT][4], sF[CT];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sA[rt][e] = s_sa[wave][par][16 * rt + 4 * kk + e];
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+            sF[ct] = ldexp(1.0, s_fe[wave][par][2 * jj + ct] + s_fe[wave][par][32 + 2 * jj + ct] - 2 * kI8FS);
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                double v = (double)acc[0][rt][ct][e];
+                v = fma(v, 256.0, (double)acc[1][rt][ct][e]);
+                v = fma(v, 256.0, (double)acc[2][rt][ct][e]);
+                v = fma(v, 256.0, (double)acc[3][rt][ct][e]);
+                v = fma(v, 256.0, (double)acc[4][rt][ct][e]);
+                tot[rt][ct][e] = fma(v, sA[rt][e] * sF[ct], tot[rt][ct][e]);
+              }
+        }
+        cur = nxt;
+        nxt = nn;
+        pa = pa_n;
+        pb = pb_n;
+      }
 #undef I8_STEP
+#undef I8_NAT
 #undef I8_LDS
 #undef I8_LOAD
-      // the block's exact sums, combined and scaled into the FP64 running sums
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            double v = (double)acc[0][rt][ct][e];
-            v = fma(v, 256.0, (double)acc[1][rt][ct][e]);
-            v = fma(v, 256.0, (double)acc[2][rt][ct][e]);
-            v = fma(v, 256.0, (double)acc[3][rt][ct][e]);
-            v = fma(v, 256.0, (double)acc[4][rt][ct][e]);
-            tot[rt][ct][e] = fma(v, sA[rt][e] * sF[ct], tot[rt][ct][e]);
-          }
-#pragma unroll
-          for (int k = 0; k < 5; ++k) acc[k][rt][ct] = izero;
-        }
     }
-    if (!active) continue;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        const int v = lcol + ct;
-        double* out = partial + (it.prow + 64 * ch + v) * D;
+        double* out = partial + (it.prow + 64 * ch + lcol + ct) * D;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int l = 16 * rt + 4 * kk + e;   // int32 16x16 output: row 4 (lane / 16) + e
@@ -1023,9 +1132,15 @@ static void destroy_plan(dgv_batch* b, BatchPlan& P) {
   if (&P == &b->plan && b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
 }
 
-// Per cell and grid: blocks whose box contains the cell, cut into segments of
-// about `tile` pairs; items = segments x chunks of 64 instances.  With `cells`
-// given, only those cells (of every grid) are planned.
+// Per cell and grid: blocks whose box contains the cell, cut into segments;
+// items = segments x chunks of 64 instances.  Segments are global: the grid's
+// blocks in (a, b) order -- the order of every cell's list -- are cut into runs
+// of about `tile` pairs, and a cell's segment k is its blocks in run k.  The
+// items of one run for different cells then walk the same blocks in the same
+// order, and run side by side (the queue is ordered by run, then cell), so
+// the coefficient records of a block are fetched from memory once and served
+// from L2 to the items running with it.  With `cells` given, only those cells
+// (of every grid) are planned.
 static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std::vector<int32_t>* cells = nullptr) {
   if (int rc = batch_check_grids(b)) return rc;
   destroy_plan(b, P);
@@ -1036,6 +1151,7 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
   std::vector<int32_t> key;
   std::vector<int64_t> rptr(1, 0);
   std::vector<int32_t> rrows;
+  std::vector<std::vector<int32_t>> run_of(b->grids.size());   // per grid: the run of each block
   for (size_t gi = 0; gi < b->grids.size(); ++gi) {
     if (b->nin[gi] == 0) continue;
     const dgv_ctx* ctx = b->grids[gi];
@@ -1043,6 +1159,25 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
     const std::vector<int64_t>& cp = b->cell_ptr[gi];
     const std::vector<int32_t>& blist = b->h_blist[gi];
     const int nchunk = b->Bpad[gi] / 64;
+    {
+      std::vector<int32_t> order(pb.blocks.size());
+      std::iota(order.begin(), order.end(), 0);
+      std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        const BlockHdr &X = pb.blocks[x], &Y = pb.blocks[y];
+        for (int d = 0; d < 3; ++d) if (X.a[d] != Y.a[d]) return X.a[d] < Y.a[d];
+        for (int d = 0; d < 3; ++d) if (X.b[d] != Y.b[d]) return X.b[d] < Y.b[d];
+        return x < y;
+      });
+      run_of[gi].assign(pb.blocks.size(), 0);
+      int64_t acc = 0;
+      int32_t run = 0;
+      for (int32_t k : order) {
+        run_of[gi][k] = run;
+        acc += pb.blocks[k].npairs;
+        if (acc >= tile) { ++run; acc = 0; }
+      }
+    }
+    const std::vector<int32_t>& run = run_of[gi];
     std::vector<int64_t> todo;
     if (cells) {
       for (int32_t c : *cells)
@@ -1052,15 +1187,12 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
     }
     for (int64_t c : todo) {
       std::vector<std::pair<int32_t, int32_t>> segs;
-      int64_t s0 = cp[c], acc = 0;
-      for (int64_t k = cp[c]; k < cp[c + 1]; ++k) {
-        acc += pb.blocks[blist[k]].npairs;
-        if (acc >= tile || k + 1 == cp[c + 1]) {
+      int64_t s0 = cp[c];
+      for (int64_t k = cp[c]; k < cp[c + 1]; ++k)
+        if (k + 1 == cp[c + 1] || run[blist[k + 1]] != run[blist[k]]) {
           segs.push_back({(int32_t)s0, (int32_t)(k + 1)});
           s0 = k + 1;
-          acc = 0;
         }
-      }
       const bool fp32 = b->precision == DGV_FP32_COMP || b->precision == DGV_FP32_F64ACC;
       const int chpi = fp32 ? chunks_per_item<true>() : chunks_per_item<false>();   // INT8_SPLIT: as FP64
       for (int vc0 = 0; vc0 < nchunk; vc0 += chpi) {
@@ -1085,18 +1217,18 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
     }
   }
   const int D = b->grids[0]->g.D;
-  // keep the partial sums within 4 GiB: coarser segments otherwise
-  if (nrows * D * 8 > ((int64_t)4 << 30) && tile < (1 << 30))
+  // keep the partial sums within 32 GiB: coarser segments otherwise
+  if (nrows * D * 8 > ((int64_t)32 << 30) && tile < (1 << 30))
     return build_batch_plan(b, P, tile * 2, cells);
-  // Execution order: by the position of the item's first block in the grid's
-  // block order, then by cell.  Items that run at the same time then stream the
-  // same coefficient records for different cells (shared through L2); each
-  // item keeps its partial-sum rows, so the reduction order is unchanged.
+  // Execution order: by the item's run of blocks, then by cell.  Items that run
+  // at the same time then stream the same coefficient records for different
+  // cells (shared through L2); each item keeps its partial-sum rows, so the
+  // reduction order does not depend on the execution order.
   {
     std::vector<int64_t> key1(items.size());
     for (size_t i = 0; i < items.size(); ++i) {
       const BatchItem& it = items[i];
-      key1[i] = ((int64_t)it.grid << 40) | ((int64_t)b->h_blist[it.grid][it.b0] << 8);
+      key1[i] = ((int64_t)it.grid << 40) | ((int64_t)run_of[it.grid][b->h_blist[it.grid][it.b0]] << 8);
     }
     std::vector<size_t> order(items.size());
     std::iota(order.begin(), order.end(), (size_t)0);
