@@ -94,8 +94,8 @@ def main():
     ap.add_argument("--no-linear", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the mixed-grid batching and 33^3 precision configurations")
-    ap.add_argument("--sur-rows", type=int, default=2048, help="(f, Q) pairs generated for the surrogate")
-    ap.add_argument("--sur-steps", type=int, default=400)
+    ap.add_argument("--sur-rows", type=int, default=8192, help="(f, Q) pairs generated for the surrogate")
+    ap.add_argument("--sur-steps", type=int, default=4000)
     ap.add_argument("--profile", action="store_true", help="evaluator loops only (for rocprofv3)")
     ap.add_argument("--headline-mode", default="INT8_SPLIT", help="batched mode timed by --profile")
     ap.add_argument("--copy-probe", action="store_true",
@@ -268,7 +268,8 @@ def main():
     # ---- learned surrogate -----------------------------------------------------------------
     sur = None
     if not args.no_surrogate:
-        sur = surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single)
+        sur = surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single,
+                            results[head]["ms_per_batch"] / B if head in results else float("nan"))
 
     # ---- CPU baseline ----------------------------------------------------------------------
     cpu = None
@@ -736,80 +737,134 @@ def measured_l2(kernel: str, workload: str):
     return None
 
 
-def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single):
-    """Trains the 19683 -> 2048 x 3 -> 19683 MLP on a bounded set of (f, Q(f))
-    pairs from the batched FP32-compensated evaluator, then reports inference
-    throughput, held-out error against the direct evaluator's Q and the rate at
-    which the probe-cell judge sends rows to the direct FP64 evaluator."""
+def surrogate_leg(dgv, d, u, v, w, gw, M, rng, args, ms_single, ms_batch_vdf):
+    """The learned part in the decomposition regime.  Near equilibrium
+    Q(f) = Q(fm) + L(Df) + Q(Df) exactly (Df = f - fm, L the linearised operator
+    of PrepareFMA_DGV); the reference's run_mode 1 evaluates Q(Df) directly, here
+    the 19683 -> 2048 x 3 -> 19683 MLP gives it (Q(Df) = s^2 net(Df / s)).  The
+    training set is written by the training-data writer (perturbed Maxwellians,
+    amplitudes 1e-3..1e-1 of fm, Q by the INT8_SPLIT batch).  Reported: held-out
+    errors against the direct Q with and without the network, the judge's
+    fallback rate at 1e-3, and the cost per VDF against the direct evaluators."""
+    from dgv0d3v_direct_collision_amd import dataset
+    from dgv0d3v_direct_collision_amd.surrogate import decomposition_rows
+    import tempfile
     N = d.nnodes
     n = args.sur_rows
-    nte = max(32, n // 8)
+    nte = 256
     ntr = n - nte
-    log(f"surrogate: writing {n} (f, Q) pairs with the training-data writer (batched FP32-compensated)")
-    import tempfile
-    from dgv0d3v_direct_collision_amd import dataset
+    T = 0.8
+    fm = dataset.equilibrium(d, T)
+    t0 = time.perf_counter()
+    d.PrepareFMA_DGV(fm)
+    d.synchronize()
+    fma_s = time.perf_counter() - t0
+    log(f"surrogate: linearised operator at fm (T = {T}) in {fma_s:.2f} s; writing {n} (f, Q) pairs "
+        f"(perturbed Maxwellians, INT8_SPLIT batch)")
     tg = time.perf_counter()
     with tempfile.TemporaryDirectory(prefix="dgv_train_") as tdir:
-        meta = dataset.TrainingSetWriter(tdir, d, n, precision=dgv.DGV_FP32_COMP, batch=256, seed_base=20261207,
-                                         tile=args.tile if args.tile > 0 else 262144).run(log=log, log_every=20)
+        meta = dataset.TrainingSetWriter(tdir, d, n, precision=dgv.DGV_INT8_SPLIT, batch=256, seed_base=20261207,
+                                         tile=args.tile if args.tile > 0 else 262144, family="perturbed_maxwellian",
+                                         T=T, amplitude=(1e-3, 1e-1)).run(log=log, log_every=8)
         gen_s = time.perf_counter() - tg
         check = dataset.verify(tdir, d, nsample=4)
         _, fmap, qmap, _ = dataset.load(tdir)
         fs, qs = np.array(fmap), np.array(qmap)
         del fmap, qmap
     writer = {"rows": n, "seconds": gen_s, "rows_per_s": n / gen_s, "gpu_seconds": meta["gpu_seconds"],
-              "mb_per_s": 2 * fs.nbytes / gen_s / 1e6, "verify": check}
-    log(f"surrogate: {gen_s:.1f} s for the pairs ({n / gen_s:.0f} rows/s, GPU {meta['gpu_seconds']:.1f} s); "
-        f"verify {check}; creating the network")
-    d_f = d.device_alloc(fs.nbytes)
-    d_q = d.device_alloc(fs.nbytes)
-    d_p = d.device_alloc(fs.nbytes)
-    d.htod(d_f, fs)
-    d.htod(d_q, qs)
+              "mb_per_s": 2 * fs.nbytes / gen_s / 1e6, "verify": check, "family": meta["vdf_family"],
+              "family_params": meta["family_params"], "precision": meta["precision"]}
+    log(f"surrogate: {gen_s:.1f} s for the pairs ({n / gen_s:.0f} rows/s); verify {check}")
+    X, Y, rs = decomposition_rows(d, fs, qs)
+    R = Y * (rs * rs)[:, None]                   # Q(Df)
+    base = qs - R                                # Q(fm) + L(Df)
+    qfm = d.EvalCollisionPeriodicAPlus_DGV(fm)
+    amp = np.max(np.abs(fs - fm[None, :]), axis=1) / np.max(fm)
+    # training rows: (Df / s, Q(Df) / s^2) and their negatives (Q(-Df) = Q(Df))
+    Xtr = np.concatenate([X[:ntr], -X[:ntr]])
+    Ytr = np.concatenate([Y[:ntr], Y[:ntr]])
+    perm = rng.permutation(len(Xtr))
+    Xtr, Ytr = np.ascontiguousarray(Xtr[perm]), np.ascontiguousarray(Ytr[perm])
+    d_x = d.device_alloc(Xtr.nbytes)
+    d_y = d.device_alloc(Ytr.nbytes)
+    d.htod(d_x, Xtr)
+    d.htod(d_y, Ytr)
+    nx = len(Xtr)
+    del Xtr, Ytr
     s = dgv.Surrogate(d, [N, 2048, 2048, 2048, N], batch=256, seed=11)
-    s.set_scales(float(np.sqrt(np.mean(fs[:ntr] ** 2))), float(np.sqrt(np.mean(qs[:ntr] ** 2))))
+    s.set_scales(float(np.sqrt(np.mean(X[:ntr] ** 2))), float(np.sqrt(np.mean(Y[:ntr] ** 2))))
     tt = time.perf_counter()
     losses = []
-    for it in range(args.sur_steps):
-        r0 = int(rng.integers(0, max(1, ntr - 256 + 1)))
-        losses.append(s.train_step(d_f + 8 * N * r0, d_q + 8 * N * r0, min(256, ntr), 1e-4))
-        if it % 100 == 0:
+    steps = args.sur_steps
+    for it in range(steps):
+        r0 = int(rng.integers(0, nx // 256)) * 256
+        lr = 1e-3 if it < steps // 2 else (3e-4 if it < 3 * steps // 4 else 1e-4)
+        losses.append(s.train_step(d_x + 8 * N * r0, d_y + 8 * N * r0, 256, lr))
+        if it % 1000 == 0:
             log(f"surrogate: step {it} loss {losses[-1]:.4e}")
     d.synchronize()
     train_s = time.perf_counter() - tt
-    te = d_f + 8 * N * ntr
-    ms_inf = s.time_predict(te, nte, d_p, niter=5)
-    p = np.zeros((nte, N))
-    d.dtoh(p, d_p)
-    err = [rel_l2(p[i], qs[ntr + i]) for i in range(nte)]
-    nrmse = float(np.linalg.norm(p - qs[ntr:]) / np.linalg.norm(qs[ntr:]))
-    psi = np.stack([np.ones(N), u, v, w, u * u + v * v + w * w])
-    mom = max(float(np.max(np.abs((p[i] * gw) @ psi.T) / (np.abs(p[i] * gw) @ np.abs(psi.T)))) for i in range(nte))
-    cells = np.linspace(0, M ** 3 - 1, 8).astype(np.int32)
-    s.set_probe_cells(cells)
-    njudge = min(nte, 256)   # the judge's fallback costs one direct Q per failing row
+    d.device_free(d_x)
+    d.device_free(d_y)
+    s.set_decomposition(True)
+    d_te = d.device_alloc(8 * N * nte)
+    d_p = d.device_alloc(8 * N * nte)
+    d.htod(d_te, fs[ntr:])
+    s.predict(d_te, nte, d_p, project=False)
+    P = np.zeros((nte, N))
+    d.dtoh(P, d_p)
+    rel = lambda a, b: np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    q_te = qs[ntr:]
+    e_sur = rel(P, q_te)
+    e_base = rel(base[ntr:], q_te)
+    e_rm2 = rel(base[ntr:] - qfm[None, :], q_te)          # L(Df) alone: the reference's run_mode 2
+    e_net = rel(P - base[ntr:], R[ntr:])
+    ms_sur = s.time_predict(d_te, nte, d_p, niter=3) / nte
+    h = M // 2
+    probes = [((h + a) * M + (h + b)) * M + (h + c) for a in (-1, 1) for b in (-1, 1) for c in (-1, 1)]
+    s.set_probe_cells(probes)
     t4 = time.perf_counter()
-    res, fb, nfb = s.eval(te, njudge, d_p, 1e-3)
-    ev_s = time.perf_counter() - t4
+    res, fb, nfb = s.eval(d_te, nte, d_p, 1e-3)
+    ms_judged = 1e3 * (time.perf_counter() - t4) / nte
     s.close()
-    for ptr in (d_f, d_q, d_p):
-        d.device_free(ptr)
-    out = {"widths": [N, 2048, 2048, 2048, N], "train_rows": ntr, "test_rows": nte, "steps": args.sur_steps,
-           "batch": 256, "lr": 1e-4, "data_gen_seconds": gen_s, "training_set_writer": writer,
-           "train_seconds": train_s,
-           "train_steps_per_s": args.sur_steps / train_s, "loss_first": losses[0],
-           "loss_last20": float(np.mean(losses[-20:])),
-           "inference_vdf_per_s": nte / (ms_inf * 1e-3), "inference_ms_per_vdf": ms_inf / nte,
-           "direct_fp64_ms_per_vdf": ms_single,
-           "heldout_rel_l2_all_rows": nrmse, "heldout_rel_l2_median_row": float(np.median(err)),
-           "heldout_rel_l2_max_row": float(np.max(err)),
-           "projected_moment_rel_max": mom, "fallback_tol": 1e-3, "fallback_rows": int(nfb),
-           "judged_rows": njudge, "probe_cells": int(len(cells)), "eval_with_judge_seconds": ev_s,
+    d.device_free(d_te)
+    d.device_free(d_p)
+    classes = {}
+    for a0, a1 in ((1e-3, 1e-2), (1e-2, 3e-2), (3e-2, 1e-1 + 1e-12)):
+        m = (amp[ntr:] >= a0) & (amp[ntr:] < a1)
+        if m.any():
+            classes[f"{a0:g}-{min(a1, 1e-1):g}"] = {
+                "rows": int(m.sum()), "within_1e-3_surrogate": float(np.mean(e_sur[m] <= 1e-3)),
+                "within_1e-3_without_network": float(np.mean(e_base[m] <= 1e-3)),
+                "median_surrogate": float(np.median(e_sur[m])), "median_without_network": float(np.median(e_base[m]))}
+    kept = ~fb
+    out = {"widths": [N, 2048, 2048, 2048, N], "regime": "decomposition: Q(f) = Q(fm) + L(f - fm) + net",
+           "fm_temperature": T, "train_rows": ntr, "train_rows_with_negatives": 2 * ntr, "test_rows": nte,
+           "steps": steps, "batch": 256, "lr": "1e-3, 3e-4 after half, 1e-4 after three quarters",
+           "linearised_operator_seconds": fma_s, "training_set_writer": writer, "train_seconds": train_s,
+           "train_steps_per_s": steps / train_s, "loss_first": losses[0], "loss_last20": float(np.mean(losses[-20:])),
+           "heldout_qdf_rel_l2_median": float(np.median(e_net)),
+           "heldout_rel_l2_median": {"surrogate": float(np.median(e_sur)), "without_network": float(np.median(e_base)),
+                                     "run_mode_2_linear_only": float(np.median(e_rm2))},
+           "heldout_rel_l2_max": {"surrogate": float(np.max(e_sur)), "without_network": float(np.max(e_base))},
+           "heldout_within_1e-3": {"surrogate": float(np.mean(e_sur <= 1e-3)),
+                                   "without_network": float(np.mean(e_base <= 1e-3)),
+                                   "run_mode_2_linear_only": float(np.mean(e_rm2 <= 1e-3))},
+           "by_amplitude": classes,
+           "judge": {"tol": 1e-3, "probe_cells": probes, "fallback_rows": int(nfb), "judged_rows": nte,
+                     "kept_rows_full_grid_rel_l2_max": float(np.max(e_sur[kept])) if kept.any() else None,
+                     "ms_per_vdf_with_judge_and_fallback": ms_judged},
+           "ms_per_vdf": {"surrogate_with_linear_part": ms_sur, "direct_int8_batched": ms_batch_vdf,
+                          "direct_fp64_single": ms_single},
+           "gain_with_judge_vs_direct_int8_batched": ms_batch_vdf / ms_judged,
+           "gain_with_judge_vs_direct_fp64_single": ms_single / ms_judged,
            "note": ("full-size dataset" if n >= 100000 else
-                    "bounded dataset: the full 100k-pair set is generated with --sur-rows 100000")}
-    log(f"surrogate: loss {losses[0]:.3e} -> {out['loss_last20']:.3e}; held-out rel L2 {nrmse:.3e} (median row "
-        f"{np.median(err):.3e}); "
-        f"{out['inference_vdf_per_s']:.0f} VDF/s inference; {nfb}/{njudge} judged rows fell back at tol 1e-3")
+                    f"bounded dataset ({n} pairs): the full 100k-pair set is generated with --sur-rows 100000")}
+    log(f"surrogate: loss {losses[0]:.3e} -> {out['loss_last20']:.3e}; held-out within 1e-3: "
+        f"{np.mean(e_sur <= 1e-3):.3f} (without the network {np.mean(e_base <= 1e-3):.3f}, L(Df) alone "
+        f"{np.mean(e_rm2 <= 1e-3):.3f}); median error {np.median(e_sur):.2e}; Q(Df) learned to {np.median(e_net):.2e}")
+    log(f"surrogate: {ms_sur:.3f} ms per VDF; with the judge at 1e-3 ({nfb}/{nte} fell back) {ms_judged:.2f} ms per "
+        f"VDF against {ms_batch_vdf:.2f} (INT8 batch) and {ms_single:.2f} (FP64 single)")
     return out
 
 

@@ -358,6 +358,48 @@ __global__ void k_rows_out(const float* __restrict__ Y, int Np, int nrows, int N
   q[(int64_t)m * ld + n] = scale * (double)Y[(int64_t)m * Np + n];
 }
 
+// One block per row: D[m][n] = F[m][n] - fm[n] (D dense, N per row) and
+// rs[m] = sqrt(mean_n D[m][n]^2), summed in a fixed order.
+__global__ __launch_bounds__(256) void k_rows_minus(const double* __restrict__ F, int64_t ld, int N,
+                                                    const double* __restrict__ fm, double* __restrict__ D,
+                                                    double* __restrict__ rs) {
+  __shared__ double red[256];
+  const int m = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < N; k += 256) {
+    const double x = F[(int64_t)m * ld + k] - fm[k];
+    D[(int64_t)m * N + k] = x;
+    acc = fma(x, x, acc);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rs[m] = sqrt(red[0] / N);
+}
+
+// D[m][n] /= rs[m] (zero rows stay zero)
+__global__ void k_rows_unit(double* __restrict__ D, int nrows, int N, const double* __restrict__ rs) {
+  const int64_t n = (int64_t)nrows * N, step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const double r = rs[i / N];
+    D[i] = r > 0.0 ? D[i] / r : 0.0;
+  }
+}
+
+// Q[m][n] = rs[m]^2 Q[m][n] + Lq[m][n] + qfm[n] (Lq dense, N per row)
+__global__ void k_rows_combine(double* __restrict__ Q, int64_t ld, int nrows, int N, const double* __restrict__ rs,
+                               const double* __restrict__ Lq, const double* __restrict__ qfm) {
+  const int64_t n = (int64_t)nrows * N, step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const int64_t m = i / N, k = i - m * N;
+    const double r = rs[m];
+    Q[m * ld + k] = fma(r * r, Q[m * ld + k], Lq[i] + qfm[k]);
+  }
+}
+
 struct Chol5 {
   double L[25];
 };
@@ -520,6 +562,15 @@ struct dgv_surrogate {
   int nprobe_nodes = 0;
   double* d_res = nullptr;
   std::vector<int32_t> probe_cells;
+  // decomposition mode: Q(f) = Q(fm) + L(Df) + Q(Df) with Df = f - fm and L the
+  // context's linearised operator at fm; the network gives Q(Df) / s^2 from
+  // Df / s, s the root mean square of the row's Df (Q(Df) is quadratic in Df)
+  int decomp = 0;
+  int64_t decomp_gen = 0;        // the linearised operator's build count at set time
+  double* d_df = nullptr;        // batch x N: Df rows of a chunk, then Df / s
+  double* d_lin = nullptr;       // batch x N: L(Df) rows
+  double* d_qfm = nullptr;       // N: Q(fm)
+  double* d_rs = nullptr;        // batch: s per row
 };
 
 namespace {
@@ -577,6 +628,8 @@ int load_rows(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows) {
   return DGV_OK;
 }
 
+// The network's part of a prediction (no projection) or, with project, the
+// prediction projected onto the complement of the collision invariants.
 int predict_chunk(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows, double* d_q, int project) {
   int rc = load_rows(s, d_f, ld, nrows);
   if (rc) return rc;
@@ -594,6 +647,42 @@ int predict_chunk(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows, do
   return DGV_OK;
 }
 
+// Decomposition mode: the linearised operator must be the one the mode was set
+// with (PrepareFMA_DGV rebuilds it at a new Maxwellian, an A-array change drops it).
+int decomp_check(dgv_surrogate* s) {
+  if (!s->decomp) return DGV_OK;
+  const LinearState& L = s->ctx->lin;
+  if (!L.fma_valid || L.nprepare != s->decomp_gen)
+    return serr(s, DGV_ERR_STATE, "the linearised operator changed since the decomposition mode was set");
+  return DGV_OK;
+}
+
+// Q of nrows rows of f: the network's output, or in decomposition mode
+// Q(fm) + L(Df) + s^2 net(Df / s); then the projection when asked for.
+int predict_rows(dgv_surrogate* s, const double* d_f, int64_t ld, int nrows, double* d_q, int project) {
+  if (!s->decomp) return predict_chunk(s, d_f, ld, nrows, d_q, project);
+  if (nrows == 0) return DGV_OK;
+  dgv_ctx* c = s->ctx;
+  const int N = (int)c->g.N;
+  const int64_t n = (int64_t)nrows * N;
+  const unsigned nbk = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1 << 20));
+  hipLaunchKernelGGL(k_rows_minus, dim3(nrows), dim3(256), 0, c->stream, d_f, ld, N, c->lin.d_vec[0], s->d_df,
+                     s->d_rs);
+  SHIP(s, hipGetLastError());
+  int rc = eval_linear_dev(c, s->d_df, N, s->d_lin, N, nrows);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rows_unit, dim3(nbk), dim3(256), 0, c->stream, s->d_df, nrows, N, s->d_rs);
+  SHIP(s, hipGetLastError());
+  if ((rc = predict_chunk(s, s->d_df, N, nrows, d_q, 0))) return rc;
+  hipLaunchKernelGGL(k_rows_combine, dim3(nbk), dim3(256), 0, c->stream, d_q, ld, nrows, N, s->d_rs, s->d_lin,
+                     s->d_qfm);
+  if (project)
+    hipLaunchKernelGGL(k_project_rows, dim3(nrows), dim3(256), 0, c->stream, d_q, ld, N, c->d_u, c->d_v, c->d_w,
+                       c->d_gwts, s->chol);
+  SHIP(s, hipGetLastError());
+  return DGV_OK;
+}
+
 void free_surrogate(dgv_surrogate* s) {
   for (SurLayer& l : s->layers) {
     for (float* p : {l.W, l.mW, l.vW, l.G, l.b, l.mb, l.vb, l.gb}) hipFree(p);
@@ -607,6 +696,7 @@ void free_surrogate(dgv_surrogate* s) {
   for (int i = 0; i < 2; ++i) { hipFree(s->dZ[i]); hipFree(s->dZt[i]); }
   hipFree(s->part); hipFree(s->dbpart); hipFree(s->losspart); hipFree(s->d_loss);
   hipFree(s->d_probe_nodes); hipFree(s->d_res);
+  hipFree(s->d_df); hipFree(s->d_lin); hipFree(s->d_qfm); hipFree(s->d_rs);
   if (s->judge) dgv_batch_destroy(s->judge);
 }
 
@@ -829,11 +919,42 @@ int dgv_sur_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nro
   if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
   if (!d_f || !d_q || nrows < 0 || ld < s->ctx->g.N) return serr(s, DGV_ERR_ARG, "predict: bad argument");
   SHIP(s, hipSetDevice(s->ctx->device));
+  if (int rc = decomp_check(s)) return rc;
   for (int64_t r = 0; r < nrows; r += s->batch) {
     const int nb = (int)std::min<int64_t>(s->batch, nrows - r);
-    int rc = predict_chunk(s, d_f + r * ld, ld, nb, d_q + r * ld, project);
+    int rc = predict_rows(s, d_f + r * ld, ld, nb, d_q + r * ld, project);
     if (rc) return rc;
   }
+  return DGV_OK;
+}
+
+int dgv_sur_set_decomposition(dgv_surrogate* s, int32_t on) {
+  if (!s) { set_global_error("null surrogate"); return DGV_ERR_ARG; }
+  if (!on) {
+    s->decomp = 0;
+    return DGV_OK;
+  }
+  dgv_ctx* c = s->ctx;
+  if (s->layers.front().in != c->g.N || s->layers.back().out != c->g.N)
+    return serr(s, DGV_ERR_ARG, "decomposition mode needs node-sized input and output layers");
+  if (!c->lin.fma_valid) return serr(s, DGV_ERR_STATE, "no linearised operator (PrepareFMA)");
+  SHIP(s, hipSetDevice(c->device));
+  const size_t bytes = sizeof(double) * (size_t)s->batch * (size_t)c->g.N;
+  struct { double** p; size_t n; } bufs[] = {{&s->d_df, bytes}, {&s->d_lin, bytes},
+                                             {&s->d_qfm, sizeof(double) * (size_t)c->g.N},
+                                             {&s->d_rs, sizeof(double) * (size_t)s->batch}};
+  for (auto& b : bufs)
+    if (!*b.p && hipMalloc(b.p, b.n) != hipSuccess) {
+      *b.p = nullptr;
+      (void)hipGetLastError();
+      return serr(s, DGV_ERR_NOMEM, "hipMalloc(decomposition buffers) failed");
+    }
+  // Q(fm), the part of the split that the reference's run_mode 1 leaves out
+  int rc = eval_q_dev(c, c->lin.d_vec[0], s->d_qfm, 1.0);
+  if (rc) return rc;
+  SHIP(s, hipStreamSynchronize(c->stream));
+  s->decomp = 1;
+  s->decomp_gen = c->lin.nprepare;
   return DGV_OK;
 }
 
@@ -879,6 +1000,7 @@ int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows,
   if (!d_f || !d_q || nrows < 0 || ld < s->ctx->g.N || !(tol >= 0)) return serr(s, DGV_ERR_ARG, "eval: bad argument");
   if (!s->judge || s->nprobe_nodes == 0) return serr(s, DGV_ERR_STATE, "no probe cells set");
   SHIP(s, hipSetDevice(s->ctx->device));
+  if (int rc = decomp_check(s)) return rc;
   if (batch_stale(s->judge)) {   // the A-array changed since the probe plan: rebuild it on the current one
     const std::vector<int32_t> cl = s->probe_cells;
     int rc = dgv_sur_set_probe_cells(s, cl.data(), (int32_t)cl.size());
@@ -896,7 +1018,9 @@ int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows,
     const int nb = (int)std::min<int64_t>(s->batch, nrows - r);
     const double* F = d_f + r * ld;
     double* Q = d_q + r * ld;
-    if ((rc = predict_chunk(s, F, ld, nb, Q, 1))) return rc;
+    // the residual compares like with like: the network's Q before the
+    // projection against the direct Q; the kept rows are projected after it
+    if ((rc = predict_rows(s, F, ld, nb, Q, 0))) return rc;
     hipLaunchKernelGGL(k_rows_to_packed, dim3((unsigned)((ctx->g.N + 63) / 64), (unsigned)((nb + 63) / 64)), dim3(256),
                        0, ctx->stream, F, ld, nb, (int)ctx->g.N, jf + off, stride);
     SHIP(s, hipGetLastError());
@@ -904,6 +1028,9 @@ int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows,
     if ((rc = dgv_batch_eval_q_probe(s->judge))) return serr(s, rc, dgv_last_error(nullptr));
     hipLaunchKernelGGL(k_probe_residual, dim3(nb), dim3(256), 0, ctx->stream, Q, ld, jq + off, stride,
                        s->d_probe_nodes, s->nprobe_nodes, s->d_res);
+    SHIP(s, hipGetLastError());
+    hipLaunchKernelGGL(k_project_rows, dim3(nb), dim3(256), 0, ctx->stream, Q, ld, (int)ctx->g.N, ctx->d_u, ctx->d_v,
+                       ctx->d_w, ctx->d_gwts, s->chol);
     SHIP(s, hipGetLastError());
     SHIP(s, hipMemcpyAsync(res.data(), s->d_res, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
     SHIP(s, hipStreamSynchronize(ctx->stream));

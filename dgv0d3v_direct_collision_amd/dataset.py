@@ -32,13 +32,33 @@ def _maxwellian(u, v, w, n, ub, T):
     return n * np.exp(-((u - ub[0]) ** 2 + (v - ub[1]) ** 2 + (w - ub[2]) ** 2) / T) / (np.pi * T) ** 1.5
 
 
-def vdf_from_seed(grid, seed_base: int, row: int) -> np.ndarray:
-    """The row's VDF, normalised to unit density.  Even rows: 1-4 Maxwellians
-    (density U[0.2,1], bulk velocity U[-1,1]^3, temperature U[0.3,1]).  Odd
-    rows: two Maxwellians plus a non-negative 5 % perturbation on a random
-    tenth of the cells."""
+FAMILIES = ("mixtures", "perturbed_maxwellian")
+
+
+def equilibrium(grid, T: float) -> np.ndarray:
+    """The perturbed-Maxwellian family's fm: zero bulk velocity, temperature T,
+    unit discrete density."""
+    u, v, w, gw = grid.nodes_u, grid.nodes_v, grid.nodes_w, grid.nodes_gwts
+    fm = _maxwellian(u, v, w, 1.0, (0.0, 0.0, 0.0), T)
+    return fm / np.sum(gw * fm)
+
+
+def vdf_from_seed(grid, seed_base: int, row: int, family: str = "mixtures", T: float = 0.8,
+                  amplitude=(1e-3, 1e-1)) -> np.ndarray:
+    """The row's VDF.  Family "mixtures", normalised to unit density: even rows
+    1-4 Maxwellians (density U[0.2,1], bulk velocity U[-1,1]^3, temperature
+    U[0.3,1]); odd rows two Maxwellians plus a non-negative 5 % perturbation on a
+    random tenth of the cells.  Family "perturbed_maxwellian" (the decomposition
+    regime): equilibrium(grid, T) plus an invariant-free polynomial perturbation
+    of log-uniform amplitude in the given range (workloads.perturbed_maxwellian_vdf)."""
     rng = np.random.default_rng([seed_base, row])
     u, v, w, gw = grid.nodes_u, grid.nodes_v, grid.nodes_w, grid.nodes_gwts
+    if family == "perturbed_maxwellian":
+        from .workloads import perturbed_maxwellian_vdf
+        a = float(np.exp(rng.uniform(np.log(amplitude[0]), np.log(amplitude[1]))))
+        return perturbed_maxwellian_vdf(u, v, w, gw, equilibrium(grid, T), T, rng, a)
+    if family != "mixtures":
+        raise ValueError(f"unknown VDF family {family!r}")
     if row % 2 == 0:
         f = np.zeros(len(u))
         for _ in range(int(rng.integers(1, 5))):
@@ -76,8 +96,11 @@ class TrainingSetWriter:
     """Streams `nrows` seeded (f, Q(f)) pairs of one grid into `path`."""
 
     def __init__(self, path: str, grid, nrows: int, precision=None, batch: int = 256, seed_base: int = 0,
-                 tile: int = 0):
+                 tile: int = 0, family: str = "mixtures", T: float = 0.8, amplitude=(1e-3, 1e-1)):
         from . import DGV_FP64
+        if family not in FAMILIES:
+            raise ValueError(f"unknown VDF family {family!r}")
+        self.family, self.T, self.amplitude = family, float(T), (float(amplitude[0]), float(amplitude[1]))
         self.path, self.grid, self.nrows = path, grid, int(nrows)
         self.precision = DGV_FP64 if precision is None else precision
         self.batch, self.seed_base, self.tile = min(int(batch), self.nrows), int(seed_base), tile
@@ -100,7 +123,8 @@ class TrainingSetWriter:
             b.set_tile(self.tile)
         d_f = g.device_alloc(8 * N * B)
         d_q = g.device_alloc(8 * N * B)
-        draw = lambda r0, k: np.stack([vdf_from_seed(g, self.seed_base, r) for r in range(r0, r0 + k)])
+        draw = lambda r0, k: np.stack([vdf_from_seed(g, self.seed_base, r, self.family, self.T, self.amplitude)
+                                       for r in range(r0, r0 + k)])
         t_gpu = 0.0
         try:
             with ThreadPoolExecutor(max_workers=1) as pool:
@@ -136,7 +160,8 @@ class TrainingSetWriter:
                          "box": [d.u_L, d.u_R, d.v_L, d.v_R, d.w_L, d.w_R]},
                 "aarray_fingerprint": aarray_fingerprint(g), "workload": g.workload(),
                 "precision": PRECISION_NAMES[self.precision],
-                "seed_base": self.seed_base, "vdf_family": "vdf_from_seed", "batch": B,
+                "seed_base": self.seed_base, "vdf_family": self.family,
+                "family_params": {"T": self.T, "amplitude": list(self.amplitude)}, "batch": B,
                 "seconds": secs, "gpu_seconds": t_gpu}
         with open(os.path.join(self.path, "meta.json"), "w") as fh:
             json.dump(meta, fh, indent=1)
@@ -164,9 +189,11 @@ def verify(path: str, grid, nsample: int = 8, seed: int = 0) -> dict:
     rng = np.random.default_rng(seed)
     rows = rng.choice(meta["rows"], size=min(nsample, meta["rows"]), replace=False)
     f_same, err = True, 0.0
+    fam = meta.get("vdf_family", "mixtures")
+    fp = meta.get("family_params", {"T": 0.8, "amplitude": [1e-3, 1e-1]})
     for r in rows:
         sb, ri = int(seeds[r, 0]), int(seeds[r, 1])
-        f_same &= bool(np.array_equal(vdf_from_seed(grid, sb, ri), f[r]))
+        f_same &= bool(np.array_equal(vdf_from_seed(grid, sb, ri, fam, fp["T"], tuple(fp["amplitude"])), f[r]))
         qd = grid.EvalCollisionPeriodicAPlus_DGV(np.asarray(f[r]))
         err = max(err, float(np.linalg.norm(qd - q[r]) / np.linalg.norm(qd)))
     return {"fingerprint_matches": fp_ok, "f_from_seeds": f_same, "q_rel_l2_max": err, "rows_checked": len(rows)}

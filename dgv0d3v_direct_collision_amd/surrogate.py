@@ -86,6 +86,13 @@ class Surrogate:
         self._chk(self._L.dgv_sur_predict(self._s, C.c_void_p(int(d_f)), ld or self.dgv.nnodes, nrows,
                                           C.c_void_p(int(d_q)), int(project)))
 
+    def set_decomposition(self, on: bool = True):
+        """Decomposition mode: the network maps Df = f - fm to Q(Df) and the
+        surrogate's Q(f) is L(Df) + net(Df), with fm and L the context's
+        prepared linearised operator (PrepareFMA_DGV); the reference's run_mode 1
+        split (spat_oper_mod.f90:28-68).  Train with ``decomposition_rows``."""
+        self._chk(self._L.dgv_sur_set_decomposition(self._s, int(bool(on))))
+
     def set_probe_cells(self, cells):
         c = np.ascontiguousarray(cells, np.int32)
         self._chk(self._L.dgv_sur_set_probe_cells(self._s, iptr(c), len(c)))
@@ -144,3 +151,38 @@ class Surrogate:
             self.dgv.device_free(df)
             self.dgv.device_free(dq)
         return out, res, fb, nfb
+
+
+def decomposition_rows(dgv: DGV, f_rows, q_rows=None, chunk: int = 256):
+    """Training rows of the decomposition mode, as host arrays.  With Df = f - fm
+    and s the root mean square of a row's Df: X = Df / s and, when q_rows (Q(f))
+    is given, Y = (Q(f) - Q(fm) - L(Df)) / s^2, the quadratic part Q(Df) / s^2;
+    fm and L = 2 MATMUL(., fmA) are the context's prepared linearised operator
+    (collision_mod.f90:346-359).  Returns (X, Y, s); Y is None without q_rows."""
+    f_rows = np.asarray(f_rows, np.float64)
+    fm = dgv.fm
+    Df = np.ascontiguousarray(f_rows - fm[None, :])
+    rs = np.sqrt(np.mean(Df * Df, axis=1))
+    inv = np.where(rs > 0, 1.0 / np.where(rs > 0, rs, 1.0), 0.0)
+    X = Df * inv[:, None]
+    if q_rows is None:
+        return X, None, rs
+    N = dgv.nnodes
+    k = max(1, min(chunk, len(Df)))
+    Lr = np.zeros_like(Df)
+    d_in = dgv.device_alloc(8 * N * k)
+    d_out = dgv.device_alloc(8 * N * k)
+    try:
+        tmp = np.zeros((k, N))
+        for r0 in range(0, len(Df), k):
+            n = min(k, len(Df) - r0)
+            dgv.htod(d_in, np.ascontiguousarray(Df[r0:r0 + n]))
+            dgv.eval_linear_dev(d_in, d_out, nvec=n)
+            dgv.dtoh(tmp[:n], d_out)
+            Lr[r0:r0 + n] = tmp[:n]
+    finally:
+        dgv.device_free(d_in)
+        dgv.device_free(d_out)
+    qfm = dgv.EvalCollisionPeriodicAPlus_DGV(fm)
+    Y = (np.asarray(q_rows, np.float64) - qfm[None, :] - Lr) * (inv * inv)[:, None]
+    return X, Y, rs

@@ -55,3 +55,32 @@ def near_equilibrium_vdf(u, v, w, gwts, rng, amplitude):
     f0 = maxwellian(u, v, w, 1.0, rng.uniform(-0.3, 0.3, 3), rng.uniform(0.6, 1.0))
     f = f0 * (1.0 + amplitude * rng.uniform(-1, 1, len(u)))
     return f / np.sum(gwts * f)
+
+
+def invariant_free(u, v, w, gwts, fm, delta):
+    """delta minus fm * (a + b.u + c |u|^2) with the five coefficients chosen so
+    that the result has no density, momentum or energy (sum_i w_i psi_k(i) x_i = 0
+    for psi = 1, u, v, w, |u|^2): fm + result keeps fm's moments."""
+    psi = np.stack([np.ones(len(u)), u, v, w, u * u + v * v + w * w])
+    G = (psi * (gwts * fm)) @ psi.T
+    lam = np.linalg.solve(G, psi @ (gwts * delta))
+    return delta - fm * (lam @ psi)
+
+
+def perturbed_maxwellian_vdf(u, v, w, gwts, fm, T, rng, amplitude, degree=4):
+    """fm + delta for the decomposition regime: delta = fm p(u) with p a random
+    polynomial in (u, v, w) / sqrt(T) of total degree 1..degree (N(0,1)
+    coefficients on the monomials), made invariant-free (invariant_free) and
+    scaled to max|delta| = amplitude max fm.  The rows of one fm share its
+    density, bulk velocity and temperature, so f - fm is the Df of the
+    reference's run_mode 1 split at that Maxwellian."""
+    s = 1.0 / np.sqrt(T)
+    x, y, z = u * s, v * s, w * s
+    p = np.zeros(len(u))
+    for a in range(degree + 1):
+        for b in range(degree + 1 - a):
+            for c in range(degree + 1 - a - b):
+                if 1 <= a + b + c:
+                    p += rng.standard_normal() * x ** a * y ** b * z ** c
+    d = invariant_free(u, v, w, gwts, fm, fm * p)
+    return fm + d * (amplitude * np.max(fm) / np.max(np.abs(d)))

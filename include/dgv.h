@@ -389,14 +389,28 @@ int dgv_sur_train_step(dgv_surrogate* s, const double* d_f, const double* d_q, i
 int dgv_sur_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q,
                     int32_t project);
 /* The direct evaluator as judge and fallback: on the probe cells the batched
- * FP64 engine gives each row's residual ||q_sur - q_direct|| / ||q_direct||;
- * rows whose residual exceeds tol receive the direct FP64 Q on every node.
+ * FP64 engine gives each row's residual ||q_sur - q_direct|| / ||q_direct||,
+ * with q_sur the surrogate's Q before the projection; rows within tol receive
+ * the projected surrogate Q, the others the direct FP64 Q on every node.
  * residual and fallback (host, nrows entries) may be NULL. */
 int dgv_sur_set_probe_cells(dgv_surrogate* s, const int32_t* cells, int32_t ncells);
 int dgv_sur_eval(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q, double tol,
                  double* residual, int32_t* fallback, int64_t* nfallback);
 int dgv_sur_time_predict(dgv_surrogate* s, const double* d_f, int64_t ld, int64_t nrows, double* d_q,
                          int32_t niter, double* ms_per_call);
+/* Decomposition mode (on = 1): Q(f) = Q(fm) + L(Df) + Q(Df) exactly, with
+ * Df = f - fm, fm and L = 2 MATMUL(., fmA) the context's prepared linearised
+ * operator (dgv_prepare_fma; collision_mod.f90:346-359) and Q(fm) evaluated
+ * once by this call.  This is the reference's run_mode 1 split
+ * (spat_oper_mod.f90:28-68) plus Q(fm), which run_mode 1 leaves out.  The
+ * network gives the quadratic part from the direction of Df: with s the root
+ * mean square of a row's Df, Q(Df) = s^2 net(Df / s).  dgv_sur_predict,
+ * dgv_sur_eval and dgv_sur_time_predict then take rows of f; the judge and the
+ * fallback still evaluate Q(f) directly.  Training takes the rows
+ * (Df / s, (Q(f) - Q(fm) - L(Df)) / s^2).  DGV_ERR_STATE without a prepared
+ * operator, or (at predict / eval) when it was rebuilt or dropped since this
+ * call.  on = 0 turns the mode off. */
+int dgv_sur_set_decomposition(dgv_surrogate* s, int32_t on);
 
 /* ---- stream / timing helpers ----------------------------------------------------- */
 int dgv_ctx_synchronize(dgv_ctx* ctx);
