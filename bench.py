@@ -100,9 +100,14 @@ def main():
     ap.add_argument("--headline-mode", default="INT8_SPLIT", help="batched mode timed by --profile")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a plain device copy of 1 GiB, 5 times (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--profile-surrogate", action="store_true",
+                    help="only the surrogate's training steps and inference on the 27^3 node count (for rocprofv3)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.profile_surrogate:
+        surrogate_kernels_probe(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the evaluator runs on one GPU and has no multi-GPU path; run with --gpus 1.",
@@ -684,6 +689,40 @@ def rk3_run(d, d_f, f, nsteps, project):
     log(f"RK3 ({'projected' if project else 'plain'}): {done} steps, {out['steps_per_sec']} steps/s, "
         f"density drift {out['density_drift']:.3e}" + (f", diverged by step {diverged}" if diverged else ""))
     return out
+
+
+def surrogate_kernels_probe(args):
+    """The 19683 -> 2048 x 3 -> 19683 network's kernels alone: 100 training steps
+    and 20 inference calls on 256 random rows (the 27^3 grid's node count), for
+    per-kernel timing; prints the step and inference rates."""
+    import dgv0d3v_direct_collision_amd as dgv
+    d = dgv.DGV(args.M, 3, (-args.box, args.box))
+    N = d.nnodes
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((256, N))
+    Y = rng.standard_normal((256, N))
+    d_x = d.device_alloc(X.nbytes)
+    d_y = d.device_alloc(Y.nbytes)
+    d_p = d.device_alloc(Y.nbytes)
+    d.htod(d_x, X)
+    d.htod(d_y, Y)
+    s = dgv.Surrogate(d, [N, 2048, 2048, 2048, N], batch=256, seed=1)
+    for _ in range(5):
+        s.train_step(d_x, d_y, 256, 1e-4)
+    d.synchronize()
+    t = time.perf_counter()
+    for _ in range(100):
+        s.train_step(d_x, d_y, 256, 1e-4)
+    d.synchronize()
+    ts = (time.perf_counter() - t) / 100
+    ms = s.time_predict(d_x, 256, d_p, niter=20)
+    flop_fwd = 2.0 * 256 * (N * 2048 * 2 + 2048 * 2048 * 2)
+    log(f"surrogate kernels: training step {ts * 1e3:.3f} ms ({3 * flop_fwd / ts / 1e12:.1f} TFLOP/s over the three "
+        f"products per layer), inference {ms:.3f} ms per 256 rows ({flop_fwd / (ms * 1e-3) / 1e12:.1f} TFLOP/s)")
+    s.close()
+    for p in (d_x, d_y, d_p):
+        d.device_free(p)
+    d.close()
 
 
 def copy_probe():

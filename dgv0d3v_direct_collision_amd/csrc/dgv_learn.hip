@@ -154,8 +154,19 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a, int mode) {
   for (int i = 0; i < 16; ++i) {
     const int r = rg + 4 * i;
     const int64_t idx = (int64_t)(m0 + r) * a.N + n0 + c;
-    float s = 0.f;
-    for (int k = 0; k < a.S; ++k) s += a.part[k * MN + idx];
+    // partials in a fixed order: four running sums (k mod 4), then ((0 + 1) + (2 + 3))
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+    int k = 0;
+    for (; k + 4 <= a.S; k += 4) {
+      q0 += a.part[(int64_t)k * MN + idx];
+      q1 += a.part[(int64_t)(k + 1) * MN + idx];
+      q2 += a.part[(int64_t)(k + 2) * MN + idx];
+      q3 += a.part[(int64_t)(k + 3) * MN + idx];
+    }
+    if (k < a.S) q0 += a.part[(int64_t)k * MN + idx];
+    if (k + 1 < a.S) q1 += a.part[(int64_t)(k + 1) * MN + idx];
+    if (k + 2 < a.S) q2 += a.part[(int64_t)(k + 2) * MN + idx];
+    const float s = (q0 + q1) + (q2 + q3);
     if (mode == EPI_GELU) {
       const float z = s + a.bias[n0 + c];
       a.zout[idx] = z;
@@ -523,12 +534,15 @@ struct Gemm {
   int M, N, K, S, kchunk;
 };
 
+// Split-K: about 512 workgroups, but at least 8 K steps per workgroup, so that
+// the FP32 partials (S x M x N, summed by the epilogue) stay a small multiple of
+// the output.
 Gemm plan_gemm(int M, int N, int K, bool allow_split) {
   Gemm g{M, N, K, 1, K};
   if (allow_split) {
     const int tiles = (M / GT_M) * (N / GT_N);
     const int kt = K / GT_K;
-    int S = std::max(1, std::min(kt, (1024 + tiles - 1) / tiles));
+    int S = std::max(1, std::min(kt / 8, (512 + tiles - 1) / tiles));
     g.kchunk = ((kt + S - 1) / S) * GT_K;
     g.S = (K + g.kchunk - 1) / g.kchunk;
   }
