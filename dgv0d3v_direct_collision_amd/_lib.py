@@ -153,6 +153,8 @@ SIGNATURES = {
     "dgv_sur_set_probe_cells": (C.c_int, [_vp, _ip, _i32]),
     "dgv_sur_eval": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _dbl, _dp, _ip, _lp]),
     "dgv_sur_set_decomposition": (C.c_int, [_vp, _i32]),
+    "dgv_iphi": (C.c_int, [_vp, _dp, _dp, _i64, _dbl]),
+    "dgv_iphi_decomp": (C.c_int, [_vp, _dp, _dp, _i64, _dbl]),
     "dgv_sur_time_predict": (C.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _dp]),
     "dgv_ctx_synchronize": (C.c_int, [_vp]),
     "dgv_time_eval_q_dev": (C.c_int, [_vp, _vp, _vp, _i32, _dp, _dp]),

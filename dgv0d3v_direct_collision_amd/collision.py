@@ -409,6 +409,22 @@ class DGV:
         self._chk(self._L.dgv_eval_mixed(self._ctx, dptr(f), dptr(fm), dptr(fc), self.nnodes))
         return fc
 
+    def IphiDGV(self, f) -> np.ndarray:
+        """dgvtools_mod.f90:847-877: 2/w_i sum A f(xi) f(xi1) for every node's basis
+        function, times (mol_diam/L_inf)^2 N_inf."""
+        f = self._f(f)
+        out = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_iphi(self._ctx, dptr(f), dptr(out), self.nnodes, self.coef))
+        return out
+
+    def IphiDGV_decomp(self, f) -> np.ndarray:
+        """dgvtools_mod.f90:896-927: IphiDGV with f split at its local Maxwellian
+        fM: 2/w_i sum A (fM Df + Df fM + Df Df), Df = f - fM."""
+        f = self._f(f)
+        out = np.zeros(self.nnodes)
+        self._chk(self._L.dgv_iphi_decomp(self._ctx, dptr(f), dptr(out), self.nnodes, self.coef))
+        return out
+
     def PrepareFMA_DGV(self, fm=None):
         """collision_mod.f90:377-464: the dense linearised operator at the
         Maxwellian fm (the context's last one when fm is None), kept on the GPU."""

@@ -606,6 +606,53 @@ int dgv_ctx_get_regime(dgv_ctx* ctx, int32_t* run_mode, double* l1_err, double* 
   return DGV_OK;
 }
 
+// IphiDGV (dgvtools_mod.f90:847-877): for every node i, one basis function per
+// node, Iphi(i) = 2 sum over its records of A f(xi) f(xi1) / w_i, times coef.
+// On the shifted A-array these are the evaluator's Q of f, scaled.
+int dgv_iphi(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef) {
+  LIN_CHECK(ctx, n);
+  if (!f || !iphi) return set_error(ctx, DGV_ERR_ARG, "null array");
+  const size_t nb = sizeof(double) * n;
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
+  int rc = eval_q_dev(ctx, ctx->d_f, ctx->d_q, coef);
+  if (rc) return rc;
+  DGV_HIP(ctx, hipMemcpyAsync(iphi, ctx->d_q, nb, hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
+// IphiDGV_decomp (dgvtools_mod.f90:896-927): the same integrals with f split at
+// its local Maxwellian fM (MassCheckRec, maxwelveldist; lines 912-914): per record
+// A (fM(xi) Df(xi1) + Df(xi) fM(xi1) + Df(xi) Df(xi1)), Df = f - fM, i.e. the
+// evaluator's mixed term of (Df, fM) plus its Q of Df, times coef.
+int dgv_iphi_decomp(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef) {
+  LIN_CHECK(ctx, n);
+  if (!f || !iphi) return set_error(ctx, DGV_ERR_ARG, "null array");
+  const size_t nb = sizeof(double) * n;
+  const Grid& g = ctx->g;
+  DGV_HIP(ctx, hipSetDevice(ctx->device));
+  DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
+  double mom[5];
+  int rc = device_moments(ctx, ctx->d_f, mom);
+  if (rc) return rc;
+  double* fM = ctx->d_tmp[5];
+  double* df = ctx->d_tmp[0];
+  double* qd = ctx->d_tmp[1];
+  const unsigned nblk = stride_blocks(g.N, 256);
+  hipLaunchKernelGGL(k_maxwell, dim3(nblk), dim3(256), 0, ctx->stream, g.N, mom[4], mom[1], mom[2], mom[3], mom[0],
+                     ctx->d_u, ctx->d_v, ctx->d_w, fM);
+  hipLaunchKernelGGL(k_sub, dim3(nblk), dim3(256), 0, ctx->stream, g.N, ctx->d_f, fM, df);
+  DGV_HIP(ctx, hipGetLastError());
+  if ((rc = eval_mixed_dev(ctx, df, fM, ctx->d_q))) return rc;
+  if ((rc = eval_q_dev(ctx, df, qd, 1.0))) return rc;
+  hipLaunchKernelGGL(k_sum_scale, dim3(nblk), dim3(256), 0, ctx->stream, g.N, ctx->d_q, qd, coef, ctx->d_q);
+  DGV_HIP(ctx, hipGetLastError());
+  DGV_HIP(ctx, hipMemcpyAsync(iphi, ctx->d_q, nb, hipMemcpyDeviceToHost, ctx->stream));
+  DGV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DGV_OK;
+}
+
 int dgv_check_solution_mode(dgv_ctx* ctx, const double* f, double* df, int64_t n) {
   LIN_CHECK(ctx, n);
   if (!f) return set_error(ctx, DGV_ERR_ARG, "null array");

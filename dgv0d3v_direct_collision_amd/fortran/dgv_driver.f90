@@ -1,7 +1,8 @@
 ! dgv_driver.f90 -- serial driver in the shape of DGVblzmPL.f90's time loop,
 ! on the GPU through the dgv_gpu shim: grid, A-array, one Q(f), nsteps RK steps.
 ! Usage: dgv_driver M box rkmts nsteps dt out.bin [decomp_lev linear_lev nsteps2]
-! Writes f0, Q(f0), f after the steps (raw float64) to out.bin.  With the three
+! Writes f0, Q(f0), f after the steps, IphiDGV(f) and IphiDGV_decomp(f) (raw
+! float64) to out.bin.  With the three
 ! optional arguments it then turns on the regime switch (CheckSolutionMode_DGV
 ! before every spatial operator), runs nsteps2 more steps and appends f,
 ! run_mode and EvalCollisionLinear(f) at the last Maxwellian.
@@ -36,6 +37,7 @@ program dgv_driver
     call TimeIntegratorORK_SH_DGV(f, dt)
   end do
   write (u) f
+  write (u) IphiDGV(f), IphiDGV_decomp(f)
   if (command_argument_count() >= 9) then
     call get_command_argument(7, arg); read (arg, *) decomp_lev
     call get_command_argument(8, arg); read (arg, *) linear_lev

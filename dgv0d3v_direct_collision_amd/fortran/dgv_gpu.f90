@@ -58,6 +58,7 @@ module dgv_gpu
   public :: TimeIntegratorORK_SH_DGV, TimeIntegratorMTS_SH_DGV, PrepareMTS_SH_DGV, MassCheckRec
   public :: EvalCollisionPeriodicMixedTermsA_DGV, PrepareFMA_DGV, EvalCollisionLinear, CheckSolutionMode_DGV
   public :: MassCheckRecHighCMoments, dgv_gpu_get_history, dgv_gpu_set_history
+  public :: IphiDGV, IphiDGV_decomp
 
   interface
     integer(c_int) function c_ctx_create(grid, device, out) bind(C, name="dgv_ctx_create")
@@ -191,6 +192,22 @@ module dgv_gpu
       real(c_double), intent(in) :: f(*)
       integer(c_int64_t), value :: n
       real(c_double), intent(out) :: out(20)
+    end function
+    integer(c_int) function c_iphi(c, f, out, n, coef) bind(C, name="dgv_iphi")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), value :: n
+      real(c_double), value :: coef
+    end function
+    integer(c_int) function c_iphi_decomp(c, f, out, n, coef) bind(C, name="dgv_iphi_decomp")
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: f(*)
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), value :: n
+      real(c_double), value :: coef
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
@@ -399,6 +416,22 @@ contains
     mom3_u = o(9); mom3_v = o(10); mom3_w = o(11); mom4_u = o(12); mom4_v = o(13); mom4_w = o(14)
     mom5_u = o(15); mom5_v = o(16); mom5_w = o(17); mom6_u = o(18); mom6_v = o(19); mom6_w = o(20)
   end subroutine
+
+  ! IphiDGV (dgvtools_mod.f90:847) and IphiDGV_decomp (dgvtools_mod.f90:896): the
+  ! moments of the collision operator for every node's basis function, scaled by
+  ! (mol_diam/L_inf)**2*N_inf; the second splits f at its local Maxwellian.
+  function IphiDGV(f) result(Iphi)
+    real(DP), dimension(:), intent(in) :: f
+    real(DP), dimension(size(f)) :: Iphi
+    call check(c_iphi(ctx, f, Iphi, size(f, kind=c_int64_t), (mol_diam/L_inf)**2*N_inf), "IphiDGV")
+  end function
+
+  function IphiDGV_decomp(f) result(Iphi)
+    real(DP), dimension(:), intent(in) :: f
+    real(DP), dimension(size(f)) :: Iphi
+    call check(c_iphi_decomp(ctx, f, Iphi, size(f, kind=c_int64_t), (mol_diam/L_inf)**2*N_inf), &
+               "IphiDGV_decomp")
+  end function
 
   ! The multistep history the steppers keep (commvar f1..f4, frhs1..frhs5 in the
   ! reference): copy it out before the reference's WriteSol_DGV and back in

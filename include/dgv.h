@@ -236,6 +236,15 @@ int dgv_ctx_set_regime(dgv_ctx* ctx, int32_t run_mode, double decomp_lev, double
  * counts = {regime checks, fmA builds} */
 int dgv_ctx_get_regime(dgv_ctx* ctx, int32_t* run_mode, double* l1_err, double* l1_err_fm,
                        int64_t counts[2]);
+/* IphiDGV(f) (dgvtools_mod.f90:847-877): for every node (one basis function per
+ * node) 2/w_i sum over its records of A f(xi) f(xi1), times coef (the
+ * reference's (mol_diam/L_inf)^2 N_inf) -- on the shifted A-array, Q(f) scaled.
+ * IphiDGV_decomp(f) (dgvtools_mod.f90:896-927): the same with f split at its
+ * local Maxwellian fM (MassCheckRec, maxwelveldist):
+ * 2/w_i sum A (fM Df + Df fM + Df Df), Df = f - fM.  The context's projection
+ * setting applies to the Q evaluations (off by default, as the reference). */
+int dgv_iphi(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef);
+int dgv_iphi_decomp(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef);
 /* CheckSolutionMode_DGV(f,Df) (spat_oper_mod.f90:195-272): moments, local
  * Maxwellian, L1 deviation, regime switch (rebuilding fmA as the reference
  * does), Df = f - fm (zeros in run_mode 0; df may be NULL). */

@@ -19,6 +19,10 @@ function               reference routine restated
 ``eval_aplus_dd_list`` the same loop at listed nodes, exact products and a
                        double-double sum (Q rounded once: the conditioning-free checker)
 ``iphi``               IphiDGV (dgvtools_mod.f90:847-877)
+``iphi_decomp``        IphiDGV_decomp (dgvtools_mod.f90:896-927), given fM (numpy)
+``full_aarray``        the canonical records replicated to every node's basis function
+                       (the full A-array IphiDGV walks; shift and bounds test of
+                       dgvtools_mod.f90:1175-1247, collision_mod.f90:196-211)
 ``masscheck``          MassCheckRec (dgvtools_mod.f90:1437-1458)
 ``masscheck_high``     MassCheckRecHighCMoments (dgvtools_mod.f90:1501-1549)
 ``maxwell``            maxwelveldist (distributions_mod.f90:83-94)
@@ -270,6 +274,52 @@ def iphi(capphi, A, A_xi, A_xi1, gwts, f, coef=1.0):
     out = np.zeros(n)
     lib().orc_iphi(n, _c(capphi, np.int32), _c(A, np.float64), _c(A_xi, np.int32),
                    _c(A_xi1, np.int32), _c(gwts, np.float64), _c(f, np.float64), coef, out)
+    return out
+
+
+def full_aarray(G: Geometry, capphi, A, A_xi, A_xi1):
+    """The canonical records replicated to every node's basis function (shift and
+    bounds test applied once, dgvtools_mod.f90:1175-1247 and collision_mod.f90:
+    196-211): (capphi, A, A_xi, A_xi1) of the full A-array that IphiDGV
+    (dgvtools_mod.f90:847-877) walks, one basis function per node, 1-based ids."""
+    D = int(np.prod(G.s))
+    ash = G.ashift(capphi)
+    M = G.M[0]
+    node_cell = np.arange(G.nnodes) // D
+    cc = np.stack([node_cell // (M * M), (node_cell // M) % M, node_cell % M], 1)
+    fullcap, fa, fx, fx1 = np.zeros(G.nnodes, np.int32), [], [], []
+    for i in range(G.nnodes):
+        k = G.nodes_phican[i] - 1
+        b, n = ash[i], capphi[k]
+        d = np.array([G.nodes_dui[i], G.nodes_dvi[i], G.nodes_dwi[i]])
+        X = cc[A_xi[b:b + n] - 1] + d
+        Y = cc[A_xi1[b:b + n] - 1] + d
+        ok = np.all((X >= 0) & (X < M) & (Y >= 0) & (Y < M), axis=1)
+        nx = ((X[:, 0] * M + X[:, 1]) * M + X[:, 2]) * D + (A_xi[b:b + n] - 1) % D
+        ny = ((Y[:, 0] * M + Y[:, 1]) * M + Y[:, 2]) * D + (A_xi1[b:b + n] - 1) % D
+        fullcap[i] = ok.sum()
+        fa.append(A[b:b + n][ok])
+        fx.append(nx[ok] + 1)
+        fx1.append(ny[ok] + 1)
+    return (fullcap, np.concatenate(fa), np.concatenate(fx).astype(np.int32),
+            np.concatenate(fx1).astype(np.int32))
+
+
+def iphi_decomp(capphi, A, A_xi, A_xi1, gwts, f, fM, coef=1.0):
+    """IphiDGV_decomp (dgvtools_mod.f90:896-927) on a full A-array, given the local
+    Maxwellian fM: 2/w_i sum A (fM(xi) Df(xi1) + Df(xi) fM(xi1) + Df(xi) Df(xi1)),
+    Df = f - fM, the records summed in order (numpy, for small grids)."""
+    Df = np.asarray(f, np.float64) - np.asarray(fM, np.float64)
+    x, y = np.asarray(A_xi) - 1, np.asarray(A_xi1) - 1
+    terms = np.asarray(A) * fM[x] * Df[y] + np.asarray(A) * Df[x] * fM[y] + np.asarray(A) * Df[x] * Df[y]
+    out = np.zeros(len(capphi))
+    ct = 0
+    for i in range(len(capphi)):
+        s = 0.0
+        for t in terms[ct:ct + capphi[i]]:
+            s += t
+        out[i] = 2 * s / gwts[i] * coef
+        ct += capphi[i]
     return out
 
 

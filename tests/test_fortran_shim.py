@@ -24,6 +24,8 @@ def test_shim_compiles_with_reference_signatures(tmp_path):
                  "MassCheckRec(f, n, ubar, vbar, wbar, temp)", "EvalCollisionLinear(f, fc)",
                  "PrepareFMA_DGV()", "CheckSolutionMode_DGV(f, Df)"):
         assert "subroutine " + name in src
+    for name in ("IphiDGV(f) result(Iphi)", "IphiDGV_decomp(f) result(Iphi)"):
+        assert "function " + name in src
     subprocess.run([FC, "-O1", f"-J{tmp_path}", "-c", os.path.join(PKG, "fortran", "dgv_gpu.f90"),
                     "-o", str(tmp_path / "s.o")], check=True)
 
@@ -41,6 +43,7 @@ def test_fortran_driver_matches_python_api(tmp_path):
     data = np.fromfile(out, dtype=np.float64)
     n = 729
     f0, q0, f2 = data[:n], data[n:2 * n], data[2 * n:3 * n]
+    iphi, iphi_d = data[3 * n:4 * n], data[4 * n:5 * n]
     d = dgv.DGV(3, 3, (-1.5, 1.5))
     d.generate_aarray()
     assert rel_l2(q0, d.EvalCollisionPeriodicAPlus_DGV(f0)) <= 1e-13
@@ -49,6 +52,8 @@ def test_fortran_driver_matches_python_api(tmp_path):
     for _ in range(2):
         d.TimeIntegratorORK_SH_DGV(f, 0.01)
     assert rel_l2(f2, f) <= 1e-13
+    assert rel_l2(iphi, d.IphiDGV(f2)) <= 1e-13
+    assert rel_l2(iphi_d, d.IphiDGV_decomp(f2)) <= 1e-13
     d.close()
 
 
@@ -66,7 +71,7 @@ def test_fortran_driver_regime_switch_matches_python_api(tmp_path):
     assert r.returncode == 0 and out.exists(), r.stdout + r.stderr
     data = np.fromfile(out, dtype=np.float64)
     n = 729
-    f0, f2, f5, mode, lin = data[:n], data[2 * n:3 * n], data[3 * n:4 * n], data[4 * n], data[4 * n + 1:]
+    f0, f2, f5, mode, lin = data[:n], data[2 * n:3 * n], data[5 * n:6 * n], data[6 * n], data[6 * n + 1:]
     d = dgv.DGV(3, 3, (-1.5, 1.5))
     d.generate_aarray()
     f = f0.copy()

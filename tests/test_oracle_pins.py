@@ -140,28 +140,26 @@ def test_full_a_equivalence_with_iphi(small):
     # IphiDGV (dgvtools_mod.f90:847-877) over the canonical records replicated
     # to every cell (shift + bounds test applied once) equals the shifted evaluator
     G, (capphi, A, xi, xi1), f = small
-    D = int(np.prod(G.s))
-    ash = G.ashift(capphi)
-    M = G.M[0]
-    node_cell = (np.arange(G.nnodes) // D)
-    cc = np.stack([node_cell // (M * M), (node_cell // M) % M, node_cell % M], 1)
-    fullcap, fa, fx, fx1 = np.zeros(G.nnodes, np.int32), [], [], []
-    for i in range(G.nnodes):
-        k = G.nodes_phican[i] - 1
-        b, n = ash[i], capphi[k]
-        d = np.array([G.nodes_dui[i], G.nodes_dvi[i], G.nodes_dwi[i]])
-        X = cc[xi[b:b + n] - 1] + d
-        Y = cc[xi1[b:b + n] - 1] + d
-        ok = np.all((X >= 0) & (X < M) & (Y >= 0) & (Y < M), axis=1)
-        nx = ((X[:, 0] * M + X[:, 1]) * M + X[:, 2]) * D + (xi[b:b + n] - 1) % D
-        ny = ((Y[:, 0] * M + Y[:, 1]) * M + Y[:, 2]) * D + (xi1[b:b + n] - 1) % D
-        fullcap[i] = ok.sum()
-        fa.append(A[b:b + n][ok]); fx.append(nx[ok] + 1); fx1.append(ny[ok] + 1)
-    fa, fx, fx1 = np.concatenate(fa), np.concatenate(fx).astype(np.int32), np.concatenate(fx1).astype(np.int32)
+    fullcap, fa, fx, fx1 = oracle.full_aarray(G, capphi, A, xi, xi1)
     q_full = oracle.iphi(fullcap, fa, fx, fx1, G.nodes_gwts, f)
     q = oracle.eval_aplus(G, capphi, A, xi, xi1, f)
     # same records in the same order, products grouped differently: rounding only
     assert np.linalg.norm(q_full - q) <= 1e-13 * np.linalg.norm(q)
+
+
+def test_iphi_decomp_is_the_split_of_iphi(small):
+    # IphiDGV_decomp (dgvtools_mod.f90:896-927) sums A (fM Df + Df fM + Df Df) per
+    # record: with fM = 0 it is IphiDGV of f, and in general IphiDGV(f) - IphiDGV(fM)
+    # (Q is a symmetric bilinear form); a dropped or doubled term fails one of them
+    G, (capphi, A, xi, xi1), f = small
+    full = oracle.full_aarray(G, capphi, A, xi, xi1)
+    gw = G.nodes_gwts
+    fM = 0.3 * np.cos(np.arange(G.nnodes)) ** 2
+    d0 = oracle.iphi_decomp(*full, gw, f, np.zeros_like(f), 2.0)
+    assert np.linalg.norm(d0 - oracle.iphi(*full, gw, f, 2.0)) <= 1e-14 * np.linalg.norm(d0)
+    d1 = oracle.iphi_decomp(*full, gw, f, fM, 2.0)
+    diff = oracle.iphi(*full, gw, f, 2.0) - oracle.iphi(*full, gw, fM, 2.0)
+    assert np.linalg.norm(d1 - diff) <= 1e-12 * np.linalg.norm(oracle.iphi(*full, gw, f, 2.0))
 
 
 def test_swap_and_variant_agreement(small):
