@@ -804,10 +804,8 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       stage_scales(cur, 0);
       bi4 r1[kI8Ring][RT], r2[kI8Ring][RT];
       bu4 rc[kI8Ring];
-#define I8_LOAD(slot, pA, pC, step)                                                                 \
+#define I8_LOAD_R(slot, RA_, RC_, step)                                                              \
   do {                                                                                              \
-    const __amdgpu_buffer_rsrc_t RA_ = ring_rsrc(pA);                                               \
-    const __amdgpu_buffer_rsrc_t RC_ = ring_rsrc(pC);                                               \
     const uint32_t sa_ = kStepA * (uint32_t)(step);                                                 \
     _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                             \
       r1[slot][rt] = __builtin_bit_cast(bi4, __builtin_amdgcn_raw_buffer_load_b128(RA_, aoff, sa_ + rt * 2048u, 0)); \
@@ -816,6 +814,7 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
     }                                                                                               \
     rc[slot] = __builtin_amdgcn_raw_buffer_load_b128(RC_, coff, 64u * (uint32_t)(step), 0);         \
   } while (0)
+#define I8_LOAD(slot, pA, pC, step) I8_LOAD_R(slot, ring_rsrc(pA), ring_rsrc(pC), step)
       // in slot order, as in the loop (the loop's waits count loads in that order)
 #pragma unroll
       for (int u = 0; u < kI8Ring; ++u) {
@@ -824,7 +823,8 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       }
       bi4 acc[5][RT][CT];
       VB fa[4], fb[4];   // f values read from LDS two steps ahead of their products
-      bi4 nat[CT];       // the x dwords of the step whose products come next
+      bi4 nat[2][CT];    // the x dwords: [step parity], those of the step whose products come next
+                         // and those of the step after it (the ring has an even number of slots)
       const char* la_lane;
       const char* lb_lane;
 #define I8_LDS(cd_)                                                                                 \
@@ -847,17 +847,16 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       // matrix cores
 #define I8_STEP(slot, next2_slot)                                                                   \
   do {                                                                                              \
-    bi4 natn_[CT];                                                                                  \
-    I8_NAT(natn_);                                                                                  \
+    I8_NAT(nat[((slot) + 1) & 1]);                                                                  \
     I8_LDS(rc[next2_slot]);                                                                         \
     _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) {                                             \
-      const bu4 u3_ = __builtin_bit_cast(bu4, nat[ct]);                                             \
+      const bu4 u3_ = __builtin_bit_cast(bu4, nat[(slot) & 1][ct]);                                 \
       const bi4 x2_ = __builtin_bit_cast(bi4, u3_ >> 8u);                                           \
       const bi4 x1_ = __builtin_bit_cast(bi4, u3_ >> 16u);                                          \
       const bi4 x0_ = __builtin_bit_cast(bi4, u3_ >> 24u);                                          \
       _Pragma("unroll") for (int rt = 0; rt < RT; ++rt) {                                           \
-        acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], nat[ct], acc[3][rt][ct], 0, 0, 0); \
-        acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r2[slot][rt], nat[ct], acc[4][rt][ct], 0, 0, 0); \
+        acc[3][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], nat[(slot) & 1][ct], acc[3][rt][ct], 0, 0, 0); \
+        acc[4][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r2[slot][rt], nat[(slot) & 1][ct], acc[4][rt][ct], 0, 0, 0); \
         acc[2][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x2_, acc[2][rt][ct], 0, 0, 0); \
         acc[1][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x1_, acc[1][rt][ct], 0, 0, 0); \
         acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x0_, acc[0][rt][ct], 0, 0, 0); \
@@ -869,7 +868,6 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                            \
       __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);                                            \
     }                                                                                               \
-    _Pragma("unroll") for (int ct = 0; ct < CT; ++ct) nat[ct] = natn_[ct];                          \
   } while (0)
       for (int j = 0; j < nb; ++j) {
         // this block's staging has landed (a full ring of loads was issued after
@@ -908,12 +906,25 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
         lb_lane = s_f[wave][1][pb] + jj * 16;
         const int nst = max(cur.nst, 1);
         I8_LDS(rc[0]);
-        I8_NAT(nat);
+        I8_NAT(nat[0]);
         I8_LDS(rc[1 % kI8Ring]);
         // every slot loads once per pass (the count of loads in flight is the
         // same on every path); past the block's end a slot loads the next
-        // block's step of its own number (again, on the last partial pass)
-        for (int st = 0; st < nst; st += kI8Ring) {
+        // block's step of its own number (again, on the last partial pass).
+        // The passes whose steps and loads all lie in this block come first,
+        // without a per-step test or a choice of block
+        int st = 0;
+        {
+          const __amdgpu_buffer_rsrc_t RA = ring_rsrc(cur.A), RC = ring_rsrc(cur.C);
+          for (const int sA = kI8Ring * (nst / kI8Ring - 1); st < sA; st += kI8Ring) {
+#pragma unroll
+            for (int u = 0; u < kI8Ring; ++u) {
+              I8_STEP(u, (u + 2) % kI8Ring);
+              I8_LOAD_R(u, RA, RC, st + u + kI8Ring);
+            }
+          }
+        }
+        for (; st < nst; st += kI8Ring) {
 #pragma unroll
           for (int u = 0; u < kI8Ring; ++u) {
             if (st + u < nst) I8_STEP(u, (u + 2) % kI8Ring);
@@ -956,6 +967,7 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
 #undef I8_NAT
 #undef I8_LDS
 #undef I8_LOAD
+#undef I8_LOAD_R
     }
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
