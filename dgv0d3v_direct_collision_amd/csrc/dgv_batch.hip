@@ -677,7 +677,11 @@ __global__ void k_i8_fixed(int64_t ncells, int32_t Bpad, const double* __restric
 //    the wave waits for all but that many;
 //  - the block headers come through the scalar cache, two blocks ahead;
 //  - the LDS reads of a step's f values are issued two steps ahead and its x
-//    dwords computed one step ahead of its products.
+//    dwords computed one step ahead of its products (the reads after the
+//    current step's products, the x dwords between them);
+//  - a block's passes whose steps and loads all lie in the block run as one
+//    unrolled body with no per-step test; only its last one or two passes
+//    choose, per slot, between this block and the next.
 constexpr int kI8Ring = 8;
 
 // s_waitcnt immediates: vector memory count n (lgkm and exp counts left free),
@@ -862,12 +866,12 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
         acc[0][rt][ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(r1[slot][rt], x0_, acc[0][rt][ct], 0, 0, 0); \
       }                                                                                             \
     }                                                                                               \
-    /* interleave: the DS reads first, then one product per three VALU instructions */             \
-    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);                                              \
+    /* interleave: one product per three VALU instructions, then the DS reads */                   \
     _Pragma("unroll") for (int g_ = 0; g_ < 5 * RT * CT; ++g_) {                                    \
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                            \
       __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);                                            \
     }                                                                                               \
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);                                              \
   } while (0)
       for (int j = 0; j < nb; ++j) {
         // this block's staging has landed (a full ring of loads was issued after
