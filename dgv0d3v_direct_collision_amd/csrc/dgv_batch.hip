@@ -1819,13 +1819,17 @@ int dgv_batch_time_eval(dgv_batch* b, int32_t niter, double* ms_per_call, double
 int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
   if (!b) { set_global_error("null batch"); return DGV_ERR_ARG; }
   BHIP(b, hipSetDevice(b->grids[0]->device));
-  const int32_t tiles[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
+  // small segments suit the INT8 kernel (16 K pairs: 2 % faster than 256 K at
+  // 27^3), large ones the FP64 kernel; a candidate whose plan cannot be built
+  // (too many work items) is skipped
+  const int32_t tiles[] = {1 << 14, 1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21};
   const int32_t wpcs[] = {4, 8};
   int32_t bt = 0, bw = 0;
   double bms = 1e300;
   bool warm = false;
   for (int32_t t : tiles) {
     int rc = build_batch_plan(b, b->plan, t);
+    if (rc == DGV_ERR_UNSUPPORTED) continue;
     if (rc) return rc;
     for (int32_t w : wpcs) {
       b->waves_per_cu = w;
@@ -1839,6 +1843,7 @@ int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms) {
       if (ms < bms) { bms = ms; bt = t; bw = w; }
     }
   }
+  if (bt == 0) return berr(b, DGV_ERR_UNSUPPORTED, "autotune: no segment size gives a plan");
   b->waves_per_cu = bw;
   int rc = build_batch_plan(b, b->plan, bt);
   if (rc) return rc;

@@ -347,6 +347,8 @@ int dgv_batch_eval_q(dgv_batch* b);
 /* one RK step (rkmts 1..3) of every instance, optionally hipGraph-replayed */
 int dgv_batch_rk_run(dgv_batch* b, double dt, double coef, int32_t rkmts, int64_t nsteps,
                      int use_graph);
+/* times segment sizes 2^14..2^21 pairs and 4 or 8 waves per CU, keeps the
+ * fastest (best = its segment size, best_ms = its time per evaluation) */
 int dgv_batch_autotune(dgv_batch* b, int32_t* best, double* best_ms);
 /* segment size (pairs; 0 = default) and persistent-grid size (waves per CU;
  * 0 = keep) of the batched evaluation */

@@ -227,3 +227,24 @@ def test_two_nodes_per_dimension_batch_matches_single(precision, tol):
     b.close()
     for g in gs:
         g.close()
+
+
+@pytest.mark.parametrize("precision", ["FP64", "INT8_SPLIT"])
+def test_autotune_keeps_q(grids, precision):
+    """dgv_batch_autotune times segment sizes 2^14..2^21 and keeps one of them;
+    Q after it agrees with Q from the default plan (sums regrouped: 1e-12)."""
+    import dgv0d3v_direct_collision_amd as dgv
+    rng = np.random.default_rng(13)
+    which = np.array([0, 1] * 20, np.int32)
+    b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + precision))
+    for i, g in enumerate(which):
+        b.set_f(i, vdf(grids[g], rng, 2))
+    b.eval_q()
+    q0 = [b.get_q(i) for i in range(len(which))]
+    best, ms = b.autotune()
+    print(f"[batch] {precision} autotune: {best} pairs per segment, {ms:.3f} ms", flush=True)
+    assert best in [1 << k for k in range(14, 22)] and ms > 0
+    b.eval_q()
+    for i in range(len(which)):
+        assert rel_l2(b.get_q(i), q0[i]) <= 1e-12
+    b.close()
