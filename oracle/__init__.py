@@ -47,7 +47,11 @@ Index conventions are the reference's: node ids inside ``A_xi``/``A_xi1`` and ce
 ids are 1-based.
 
 Parity pins (what fixes each function, since the reference ships no tests or
-golden vectors) live in ``tests/test_oracle_*.py``.
+golden vectors) live in ``tests/test_oracle_*.py``.  ``iphi``, ``iphi_decomp``,
+``masscheck``, ``masscheck_high``, ``maxwell``, ``trunc_threshold`` and
+``trunc_radius`` are also held bit for bit to the reference's own routines,
+compiled unmodified into ``_ref/`` (``ref.py``, ``tests/test_ref_pins.py``), and
+to their stored outputs (``tests/golden/``, ``tests/test_golden_ref.py``).
 """
 from __future__ import annotations
 
@@ -311,13 +315,14 @@ def iphi_decomp(capphi, A, A_xi, A_xi1, gwts, f, fM, coef=1.0):
     Df = f - fM, the records summed in order (numpy, for small grids)."""
     Df = np.asarray(f, np.float64) - np.asarray(fM, np.float64)
     x, y = np.asarray(A_xi) - 1, np.asarray(A_xi1) - 1
-    terms = np.asarray(A) * fM[x] * Df[y] + np.asarray(A) * Df[x] * fM[y] + np.asarray(A) * Df[x] * Df[y]
+    a = np.asarray(A, np.float64)
+    t1, t2, t3 = a * fM[x] * Df[y], a * Df[x] * fM[y], a * Df[x] * Df[y]
     out = np.zeros(len(capphi))
     ct = 0
     for i in range(len(capphi)):
         s = 0.0
-        for t in terms[ct:ct + capphi[i]]:
-            s += t
+        for j in range(ct, ct + capphi[i]):   # the statement's left-to-right additions (:919)
+            s = ((s + t1[j]) + t2[j]) + t3[j]
         out[i] = 2 * s / gwts[i] * coef
         ct += capphi[i]
     return out
