@@ -49,13 +49,27 @@ def fortran_e(x: float, w: int, d: int) -> str:
     return "*" * w if len(s) > w else s.rjust(w)
 
 
+def list_directed(text: str) -> str:
+    """A list-directed WRITE (*) of character items (concatenated in ``text``) as
+    the LLVM Flang runtime lays it out: records of at most 79 columns, each a
+    blank and the next 78 characters (the reference's own readwrite.f90
+    routines, built with amdflang, write their headers so)."""
+    lines = []
+    rest = text
+    while True:
+        lines.append(" " + rest[:78])
+        rest = rest[78:]
+        if not rest:
+            return "\n".join(lines)
+
+
 # field formats of WriteErrHighCMoments_DGV: time, then 20 moments
 _MACRO_D = [14] + [15] * 8 + [14] + [15] * 5 + [14] + [15] * 5
-_MACRO_HEAD = (" This is the file containing the values of density, average velocity and temperature, "
-               "and the time stamp:",
-               " Time , Ndens , Ubar , Vbar , Wbar , T , Tx , Ty , Tz , (u-ubar)^3 ,(v-vbar)^3 , (w-wbar)^3 , "
-               "(u-ubar)^4 , (v-vbar)^4 , (w-wbar)^4, (u-ubar)^5 , (v-vbar)^5 , (w-wbar)^5 , (u-ubar)^6 , "
-               "(v-vbar)^6 , (w-wbar)^6")
+_MACRO_HEAD = (list_directed("This is the file containing the values of density, average velocity and "
+                             "temperature, and the time stamp:"),
+               list_directed("Time , Ndens , Ubar , Vbar , Wbar , T , Tx , Ty , Tz , (u-ubar)^3 ,(v-vbar)^3 , "
+                             "(w-wbar)^3 , (u-ubar)^4 , (v-vbar)^4 , (w-wbar)^4, (u-ubar)^5 , (v-vbar)^5 , "
+                             "(w-wbar)^5 , (u-ubar)^6 , (v-vbar)^6 , (w-wbar)^6"))
 
 
 def macro_line(time: float, moments: Sequence[float]) -> str:
@@ -122,9 +136,9 @@ class L1Track:
 
     def write(self) -> None:
         with open(self.path, "w") as fh:
-            fh.write(" This file contains the values of relative L1 error of deviation from local maxwellian "
-                     "and the time stamp:\n")
-            fh.write(" Time\t\tL1_err\n")
+            fh.write(list_directed("This file contains the values of relative L1 error of deviation from local "
+                                   "maxwellian and the time stamp:") + "\n")
+            fh.write(list_directed("Time\t\tL1_err") + "\n")
             for t, a in zip(self.t, self.a):
                 fh.write(fortran_f(t, 18, 14) + " ," + fortran_f(a, 18, 15) + "\n")
 
@@ -209,7 +223,7 @@ class MomDerivLog:
         for suffix, head, cols in (("_dermom.txt", _DERIV_HEAD[0], range(0, 21)),
                                    ("_relaxtimes.txt", _DERIV_HEAD[1], [0] + list(range(21, 41)))):
             with open(self.base + suffix, "w") as fh:
-                fh.write(" " + head + "\n")
+                fh.write(list_directed(head) + "\n")
                 fh.write(_MACRO_HEAD[1] + "\n")
                 for r in self.rows:
                     fh.write(" ,".join(fortran_e(r[c], 18, 6) for c in cols) + "\n")

@@ -10,6 +10,11 @@ every node (80 % of the prefiltered pairs dropped, seed 5), a seeded bimodal f,
 200 random Maxwellian arguments.  tests/test_golden_ref.py checks the oracle
 against the stored outputs wherever the reference is absent.
 
+It also writes tests/golden/ref_files/: the reference's own file names and
+files (A-array file, _sol.dat for rkmts 1..5 and the initial-data form, the
+macro-parameter, L1 and moment-derivative logs) from readwrite.f90, with their
+inputs, against which tests/test_golden_ref.py checks the library's writers.
+
 Run from the repository root: python tests/golden/make_ref_vectors.py
 """
 import os
@@ -68,5 +73,85 @@ def main():
     print(f"wrote {OUT}: {len(A)} records, {os.path.getsize(OUT)} bytes")
 
 
+
+IO_DIR = os.path.join(ROOT, "tests", "golden", "ref_files")
+
+
+def io_vectors():
+    """Files written by the reference's readwrite routines (and the names its
+    MakeBaseName routines build), with their inputs, in tests/golden/ref_files/:
+    tests/test_golden_ref.py checks the library's writers against them."""
+    import json
+    import tempfile
+    from dgv0d3v_direct_collision_amd import driver_io  # noqa: F401  (checks the package imports)
+    os.makedirs(IO_DIR, exist_ok=True)
+    rng = np.random.default_rng(11)
+    inputs = {}
+    names = []
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            os.makedirs("o", exist_ok=True)
+            for M, k_c, N_x, d, base, aop in ((3, 1, 1, "", "sol", "Aop"), (5, 2, 3, "out/", "run17", "A"),
+                                              (33, 3, 100, "a/b/", "long_base", "Ab")):
+                ref.set_names(d, base, aop, k_c, 3, N_x, M)
+                names.append({"M": M, "k_c": k_c, "N_x": N_x, "dir": d, "base": base, "aop": aop,
+                              "sol_base": ref.base_name(ref.BASE),
+                              "chunks": [ref.base_name(ref.AOPER_CHUNK, ch) for ch in (0, 7, 123)]})
+            ref.set_names("o/", "run", "A", 1, 3, 1, 3)
+            b = ref.base_name(ref.BASE)
+            # A-array file
+            mm, nn = 27, 300
+            phi = np.sort(rng.integers(1, mm + 1, nn)).astype(np.int32)
+            arr = (np.bincount(phi - 1, minlength=mm).astype(np.int32), rng.standard_normal(nn),
+                   rng.integers(1, 500, nn).astype(np.int32), None, phi)
+            arr = (arr[0], arr[1], arr[2], (arr[2] + rng.integers(1, 50, nn)).astype(np.int32), phi)
+            ref.write_aarray(*arr)
+            os.replace(b + "_Aarrs.dat", os.path.join(IO_DIR, "aarrs.dat"))
+            for k, v in zip(("capphi", "A", "A_xi", "A_xi1", "A_phi"), arr):
+                inputs["aarr_" + k] = v
+            # _sol.dat, rkmts 1..5 and the initial-data form
+            n = 27
+            f, hist = rng.random(n), rng.standard_normal((9, n))
+            inputs["sol_f"], inputs["sol_hist"] = f, hist
+            for k in range(1, 6):
+                ref.write_sol("0.375", 0.375, 0.0125, k, f, hist)
+                os.replace(b + "_time0.375_sol.dat", os.path.join(IO_DIR, f"sol_k{k}.dat"))
+            ref.write_sol("i", 0.0, 0.0125, 3, f, hist, init=True)
+            os.replace(b + "_timei_sol.dat", os.path.join(IO_DIR, "sol_init.dat"))
+            # logs
+            rows = np.column_stack([np.arange(5) * 0.0125, 1 + rng.standard_normal((5, 20)) * 0.1])
+            inputs["macro_rows"] = rows
+            ref.write_macroerr(rows)
+            os.replace(b + "_macroerr.txt", os.path.join(IO_DIR, "macroerr.txt"))
+            ref.l1_reset()
+            errs = rng.uniform(1e-4, 0.3, 400)
+            inputs["l1_err"] = errs
+            for i, e in enumerate(errs):
+                ref.l1_track(0.00137 * i, 0.0, 0.5, 4, float(e))
+            os.replace(b + "_L1err.txt", os.path.join(IO_DIR, "l1err.txt"))
+            G = oracle.Geometry(3, 3, (-2.5, 2.5))
+            ref.set_nodes(G.nodes_u, G.nodes_v, G.nodes_w, G.nodes_gwts)
+            ref.momderiv_reset()
+            f1s, r1s = [], []
+            for i in range(3):
+                f1 = bimodal(G.nodes_u, G.nodes_v, G.nodes_w, seed=40 + i)
+                r1 = 0.05 * rng.standard_normal(G.nnodes) * f1
+                ref.momderiv_record(0.1 * i, f1, r1, 6, 1.0, 3)
+                f1s.append(f1), r1s.append(r1)
+            inputs["md_f1"], inputs["md_frhs1"], inputs["md_rows"] = np.array(f1s), np.array(r1s), ref.momderiv_rows()
+            ref.momderiv_write()
+            os.replace(b + "_dermom.txt", os.path.join(IO_DIR, "dermom.txt"))
+            os.replace(b + "_relaxtimes.txt", os.path.join(IO_DIR, "relaxtimes.txt"))
+        finally:
+            os.chdir(cwd)
+    np.savez_compressed(os.path.join(IO_DIR, "inputs.npz"), **inputs)
+    with open(os.path.join(IO_DIR, "names.json"), "w") as fh:
+        json.dump(names, fh, indent=1)
+    print(f"wrote {IO_DIR}")
+
+
 if __name__ == "__main__":
     main()
+    io_vectors()
