@@ -61,21 +61,35 @@ struct BatchItem {
   int64_t prow;                // first partial row
 };
 
-// The INT8 kernel's eight work queues: items beg[q] .. beg[q + 1] - 1.  A
+// The batched kernels' eight work queues: items beg[q] .. beg[q + 1] - 1.  A
 // queue holds whole runs of blocks (every cell's items of a run), so the
 // workgroups that start on it -- those of one XCD, which the dispatcher fills
 // round-robin by workgroup index -- share that run's records in their L2.
-constexpr int kI8Queues = 8;
+constexpr int kQueues = 8;
 struct QueueBounds {
-  int32_t beg[kI8Queues + 1];
+  int32_t beg[kQueues + 1];
 };
+
+// The next item for a workgroup (called by one thread; q and left are that
+// thread's and start at blockIdx.x % kQueues and kQueues): its own queue
+// first, then the others in turn once it is empty.  An empty queue stays empty,
+// so a workgroup ends after at most one failed take per queue; -1 = no work left.
+__device__ __forceinline__ int take_item(int* __restrict__ counter, const QueueBounds& QB, int& q, int& left) {
+  while (left > 0) {
+    const int k = atomicAdd(counter + q, 1);
+    if (k < QB.beg[q + 1] - QB.beg[q]) return QB.beg[q] + k;
+    q = (q + 1) % kQueues;
+    --left;
+  }
+  return -1;
+}
 
 // One work decomposition: the items of the queue and the fixed-order reduction
 // of their partial rows.  A batch keeps the plan over all cells and, on
 // request, a probe plan over a chosen set of cells.
 struct BatchPlan {
   int32_t tile = 0;
-  QueueBounds queues{};            // INT8_SPLIT: queue q holds items queues.beg[q] .. queues.beg[q + 1] - 1
+  QueueBounds queues{};            // queue q holds items queues.beg[q] .. queues.beg[q + 1] - 1
   std::vector<BatchItem> items;
   BatchItem* d_items = nullptr;
   int64_t nrows = 0;
@@ -201,7 +215,7 @@ __global__ void k_pairs_to_f32(int64_t npairs, int D, const double* __restrict__
 }
 
 __global__ void k_reset_counter(int* c) {
-  if (blockIdx.x == 0 && threadIdx.x < kI8Queues) c[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kQueues) c[threadIdx.x] = 0;
 }
 
 // Knuth's two-sum: s + e == a + b exactly.
@@ -241,7 +255,7 @@ __device__ __forceinline__ int f32_row_fn(int rt, int i) {
 // sums) or DGV_FP32_F64ACC (the same FP32 products and sums, accumulated in
 // FP64 instead: two VALU instructions per element against seven).
 template <int D, int MODE>
-__global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchItem* __restrict__ items, int nitems,
+__global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchItem* __restrict__ items, const QueueBounds QB,
                                                                    const BatchGridDev* __restrict__ grids,
                                                                    const double* __restrict__ f64,
                                                                    const float* __restrict__ f32,
@@ -278,12 +292,13 @@ __global__ __launch_bounds__(64 * kWavesPerItem) void k_batch_mfma(const BatchIt
   for (int rt = 0; rt < RT; ++rt) loffb64[rt] = (uint32_t)((kk * (D + 1) + min(16 * rt + jj, D)) * 8);
   const uint32_t coffb64 = (uint32_t)((kk * (D + 1) + D) * 8 + 4);   // the LDS offsets (pair_code_bits)
   const V4 vzero = V4{0, 0, 0, 0};
+  int q = blockIdx.x % kQueues, left = kQueues;   // this workgroup's XCD queue first
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1);
+    if (tid == 0) s_item = take_item(counter, QB, q, left);
     __syncthreads();
     const int id = __builtin_amdgcn_readfirstlane(s_item);   // the workgroup's item, uniform
     __syncthreads();
-    if (id >= nitems) return;
+    if (id < 0) return;
     const BatchItem it = items[id];
     const BatchGridDev G = grids[it.grid];
     const bool active = ch < it.nvc;
@@ -741,24 +756,9 @@ __global__ __launch_bounds__(256) void k_batch_i8(const BatchItem* __restrict__ 
   const uint32_t aoff = (uint32_t)lane * 32u;      // the lane's 8 digit dwords of a step's row tile
   const uint32_t coff = (uint32_t)kk * 16u;        // its lane group's 4 codes of a step
   const bi4 izero = bi4{0, 0, 0, 0};
-  // the queue of this workgroup's XCD first, then the others in turn once it
-  // is empty (an empty queue stays empty, so every workgroup ends after at most
-  // one failed take per queue)
-  int q = blockIdx.x % kI8Queues, left = kI8Queues;
+  int q = blockIdx.x % kQueues, left = kQueues;   // this workgroup's XCD queue first
   for (;;) {
-    if (tid == 0) {
-      int take = -1;
-      while (left > 0) {
-        const int k = atomicAdd(counter + q, 1);
-        if (k < QB.beg[q + 1] - QB.beg[q]) {
-          take = QB.beg[q] + k;
-          break;
-        }
-        q = (q + 1) % kI8Queues;
-        --left;
-      }
-      s_item = take;
-    }
+    if (tid == 0) s_item = take_item(counter, QB, q, left);
     __syncthreads();
     const int id = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
@@ -1281,12 +1281,12 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
     std::vector<BatchItem> sorted(items.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
     items.swap(sorted);
-    // INT8_SPLIT: whole runs to eight queues, each run to the queue with the
-    // fewest items so far; a queue keeps its runs in order
-    for (int q = 0; q <= kI8Queues; ++q) P.queues.beg[q] = (int32_t)items.size();
-    if (b->precision == DGV_INT8_SPLIT && !items.empty()) {
-      std::vector<std::vector<BatchItem>> qs(kI8Queues);
-      std::vector<int64_t> load(kI8Queues, 0);
+    // whole runs to eight queues, each run to the queue with the fewest items
+    // so far; a queue keeps its runs in order
+    for (int q = 0; q <= kQueues; ++q) P.queues.beg[q] = (int32_t)items.size();
+    if (!items.empty()) {
+      std::vector<std::vector<BatchItem>> qs(kQueues);
+      std::vector<int64_t> load(kQueues, 0);
       for (size_t i = 0; i < items.size();) {
         size_t j = i;
         const int64_t k = key1[order[i]];
@@ -1297,11 +1297,11 @@ static int build_batch_plan(dgv_batch* b, BatchPlan& P, int32_t tile, const std:
         i = j;
       }
       items.clear();
-      for (int q = 0; q < kI8Queues; ++q) {
+      for (int q = 0; q < kQueues; ++q) {
         P.queues.beg[q] = (int32_t)items.size();
         items.insert(items.end(), qs[q].begin(), qs[q].end());
       }
-      P.queues.beg[kI8Queues] = (int32_t)items.size();
+      P.queues.beg[kQueues] = (int32_t)items.size();
     }
   }
   P.items.swap(items);
@@ -1348,7 +1348,7 @@ static int batch_eval_dev(dgv_batch* b, const BatchPlan& P, const double* d_f, d
   if (nitems > 0) {
     const dim3 blk(64 * kWavesPerItem);
 #define DGV_BATCH_LAUNCH(DD, MM)                                                                          \
-  hipLaunchKernelGGL((k_batch_mfma<DD, MM>), dim3(nwg), blk, 0, st, P.d_items, nitems, b->d_grids, d_f, \
+  hipLaunchKernelGGL((k_batch_mfma<DD, MM>), dim3(nwg), blk, 0, st, P.d_items, P.queues, b->d_grids, d_f, \
                      b->d_f32, P.d_partial, b->d_counter)
     if (b->precision == DGV_INT8_SPLIT) {
       if (D == 27)
@@ -1507,7 +1507,7 @@ int dgv_batch_create(dgv_ctx* const* grids, int32_t ngrids, const int32_t* grid_
     return fail(DGV_ERR_NOMEM, "hipMalloc(FP32 buffers)");
   if (i8 && (hipMalloc(&b->d_fhat, nb) || hipMalloc(&b->d_fexp, sizeof(int32_t) * std::max<int64_t>(b->total / D, 1))))
     return fail(DGV_ERR_NOMEM, "hipMalloc(fixed-point buffers)");
-  if (hipMalloc(&b->d_counter, sizeof(int) * kI8Queues)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
+  if (hipMalloc(&b->d_counter, sizeof(int) * kQueues)) return fail(DGV_ERR_NOMEM, "hipMalloc(counter)");
   {
     int64_t nmax = 1;
     for (int g = 0; g < ngrids; ++g) nmax = std::max<int64_t>(nmax, grids[g]->g.N);
