@@ -267,8 +267,8 @@ def main():
     mixed = prec33 = plain = None
     if not args.no_configs:
         plain = plain_config_leg(dgv, args)
-        mixed = mixed_grid_leg(dgv, d, rng, args)
-        prec33 = precision_leg(dgv, rng, args)
+        mixed = mixed_grid_leg(dgv, d, rng, args, {p: r["tile"] for p, r in results.items()})
+        prec33 = precision_leg(dgv, rng, args, results["INT8_SPLIT"]["tile"] if "INT8_SPLIT" in results else 0)
 
     # ---- learned surrogate -----------------------------------------------------------------
     sur = None
@@ -478,12 +478,13 @@ def plain_config_leg(dgv, args):
             "agen_seconds": agen, "density_drift": drift}
 
 
-def mixed_grid_leg(dgv, d27, rng, args):
+def mixed_grid_leg(dgv, d27, rng, args, tiles=None):
     """512 VDFs of uneven size in one persistent launch: 128 on 15^3 (5 cells,
     [-3.5,3.5]^3), 128 on 21^3 (7 cells, [-4,4]^3), 256 on 27^3 (the main grid);
     seeded 1-3 Maxwellian mixtures; FP32-compensated, FP32 with FP64
     accumulation and FP64 timed; a hipGraph-captured RK2 step of the whole
-    batch."""
+    batch.  Segment size per mode: the one the autotuner chose for it on the
+    27^3 batch (tiles), else --tile, else 262144."""
     log("mixed grids: A-arrays for 15^3 and 21^3")
     g15 = dgv.DGV(5, 3, (-3.5, 3.5))
     g21 = dgv.DGV(7, 3, (-4.0, 4.0))
@@ -499,7 +500,7 @@ def mixed_grid_leg(dgv, d27, rng, args):
         b = dgv.DGVBatch(grids, which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
-        b.set_tile(args.tile if args.tile > 0 else 262144)
+        b.set_tile(args.tile if args.tile > 0 else (tiles or {}).get(prec, 262144))
         b.eval_q()
         t = time.perf_counter()
         b.eval_q()
@@ -524,12 +525,14 @@ def mixed_grid_leg(dgv, d27, rng, args):
     return out
 
 
-def precision_leg(dgv, rng, args):
+def precision_leg(dgv, rng, args, tile=0):
     """64 near-equilibrium VDFs on 33^3 (11 cells on [-4.5,4.5]^3): a Maxwellian
     plus a seeded perturbation of relative amplitude 1e-3, 1e-2 or 1e-1;
     FP32-compensated (and FP32 with FP64 accumulation) against the FP64 path (FP64 is held to the oracle by the
     parity tests); relative L2 and max-norm errors per VDF, the conditioning
-    of each Q, and the moments after the FP64 projection."""
+    of each Q, and the moments after the FP64 projection.    Segment size: the one the autotuner chose for the INT8_SPLIT batch at 27^3
+    (tile), else --tile, else 262144.
+    """
     M = 11
     log("precision config: 33^3 A-array")
     g = dgv.DGV(M, 3, (-4.5, 4.5))
@@ -550,7 +553,7 @@ def precision_leg(dgv, rng, args):
         b = dgv.DGVBatch([g], which, getattr(dgv, "DGV_" + prec))
         for i, f in enumerate(fs):
             b.set_f(i, f)
-        b.set_tile(args.tile if args.tile > 0 else 262144)
+        b.set_tile(args.tile if args.tile > 0 else (tile or 262144))
         b.eval_q()
         t = time.perf_counter()
         b.eval_q()
