@@ -530,9 +530,9 @@ def precision_leg(dgv, rng, args, tile=0):
     plus a seeded perturbation of relative amplitude 1e-3, 1e-2 or 1e-1;
     FP32-compensated (and FP32 with FP64 accumulation) against the FP64 path (FP64 is held to the oracle by the
     parity tests); relative L2 and max-norm errors per VDF, the conditioning
-    of each Q, and the moments after the FP64 projection.    Segment size: the one the autotuner chose for the INT8_SPLIT batch at 27^3
-    (tile), else --tile, else 262144.
-    """
+    of each Q, and the moments after the FP64 projection.  Segment size: the one
+    the autotuner chose for the INT8_SPLIT batch at 27^3 (tile), else --tile,
+    else 262144."""
     M = 11
     log("precision config: 33^3 A-array")
     g = dgv.DGV(M, 3, (-4.5, 4.5))
