@@ -413,6 +413,30 @@ def cpu_baseline_leg(d, f, q_single, M, L, args):
     oracle.eval_aplus_list(G, capphi, A, xi, xi1, f, n1, nthreads=1)
     cs1 = time.perf_counter() - tc
     qps1 = len(n1) / (cs1 * N)
+    # per-node visit counts (in-grid records of the node's basis function after the
+    # shift to its cell) from a 0.2 % sample of each basis function's records: a
+    # record reaches the cells of a box, so the sampled nodes' visits and all
+    # visits follow from the sampled boxes; the latter is checked against the
+    # exact count of dgv_ctx_workload
+    vs, vt, off = 0.0, 0.0, 0
+    rs = np.random.default_rng(3)
+    ccv = np.array([(d.ccell - 1) // (M * M), ((d.ccell - 1) // M) % M, (d.ccell - 1) % M])
+    cxyz = np.stack([cells // (M * M), (cells // M) % M, cells % M], 1)
+    for n_l in counts:
+        sel = off + np.flatnonzero(rs.random(int(n_l)) < 0.002)
+        off += int(n_l)
+        a_c, b_c = (xi[sel] - 1) // D, (xi1[sel] - 1) // D
+        da = np.stack([a_c // (M * M), (a_c // M) % M, a_c % M], 1) - ccv
+        db = np.stack([b_c // (M * M), (b_c // M) % M, b_c % M], 1) - ccv
+        lo = np.maximum(0, np.maximum(-da, -db))
+        hi = np.minimum(M - 1, np.minimum(M - 1 - da, M - 1 - db))
+        vt += float(np.prod(np.maximum(hi - lo + 1, 0), axis=1).sum())
+        for c in cxyz:
+            vs += float(np.count_nonzero(np.all((lo <= c) & (c <= hi), axis=1)))
+    vt_exact = float(d.workload()["visits"])
+    qps_visits = vs / (cs * vt) if vt > 0 else 0.0
+    log(f"cpu baseline 27^3, visit-weighted: the sample's nodes make {vs / vt:.4e} of all visits (sampled records; "
+        f"all-visit estimate / exact {vt * 500 / vt_exact:.4f}): {qps_visits:.3e} Q/s")
     del A, xi, xi1
     log(f"cpu baseline 27^3: {qps:.3e} Q/s on {threads} threads ({cs:.1f} s), {qps1:.3e} Q/s on 1 core "
         f"({cs1:.1f} s for {len(n1)} nodes); GPU FP64 vs oracle on the sample: rel L2 {err:.2e}")
@@ -433,12 +457,14 @@ def cpu_baseline_leg(d, f, q_single, M, L, args):
     return {"value": qps, "unit": "VDF Q(f) evals/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "nproc": nproc,
             "value_1core": qps1,
+            "value_visit_scaled": qps_visits, "sample_visit_fraction": vs / vt if vt > 0 else None,
             "plain_15cubed": {"seconds_per_q": c15, "q_per_s": 1.0 / c15, "config_seconds_201_q": 201 * c15,
                               "threads": threads, "gpu_vs_oracle_rel_l2": err15},
             "sample": f"oracle restatement of EvalCollisionPeriodicAPlus_DGV (FP64, reference loop order, OpenMP "
                       f"threads over nodes) on the 27^3 single-VDF configuration: {len(nodes)} of {N} nodes "
                       f"({len(cells)} evenly spaced cells x all {D} basis functions), {cs:.1f} s on {threads} of "
-                      f"{nproc} host threads, scaled to N nodes (extrapolated); 1 core: {len(n1)} nodes in "
+                      f"{nproc} host threads, scaled to N nodes (extrapolated; scaled by the sample's share of in-grid visits instead: "
+                      f"{qps_visits:.3e} Q/s); 1 core: {len(n1)} nodes in "
                       f"{cs1:.1f} s; GPU FP64 vs oracle on the sample rel L2 {err:.2e}. 15^3 plain configuration: one "
                       f"full Q in {c15:.2f} s on {threads} threads (config = 201 Q)"}
 
