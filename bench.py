@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--rk-steps", type=int, default=500)
     ap.add_argument("--cpu-cells", type=int, default=48, help="cells in the CPU baseline's 27^3 sample (x 27 nodes)")
     ap.add_argument("--cpu-nodes-1core", type=int, default=27, help="nodes of the sample timed on one core")
-    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline (the box's share)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads of the CPU baseline (the host's share)")
     ap.add_argument("--tile", type=int, default=0, help="batched segment size (0: autotune)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
@@ -381,8 +381,10 @@ def cpu_baseline_leg(d, f, q_single, M, L, args):
         every canonical basis function in evenly spaced cells, scaled to all N
         nodes (per-node cost: the scan of the basis function's whole record list
         plus the in-grid visits, so a sample over all basis functions and over
-        boundary and interior cells scales by node count); on the box's core
-        share and on one core.
+        boundary and interior cells scales by node count); on the host's core
+        share and on one core; also scaled by the sample's share of in-grid
+        record visits instead (value_visit_scaled), from a 0.2 % record sample
+        whose implied total is checked against the exact visit count.
       * 15^3 (the plain configuration): one full Q over all 3375 nodes, timed end
         to end on the core share; the configuration (1 Q + 100 RK2 steps = 201 Q
         evaluations) is 201 of them, the RK2 combinations being O(N)."""

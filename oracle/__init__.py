@@ -1,6 +1,6 @@
 """CPU oracle for the direct DGV collision path -- TEST INFRASTRUCTURE ONLY.
 
-This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+This package is the oracle.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import it; the product package
 (``dgv0d3v_direct_collision_amd``) never does.  It wraps ``libdgv_oracle.so``, a
 plain-C restatement of the reference's routines (alexmalekseenko/
@@ -17,7 +17,7 @@ function               reference routine restated
 ``eval_aplus_list``    the same loop at a list of nodes (bench's CPU sample, 27^3 parity)
 ``eval_aplus_comp``    the same loop with Neumaier-compensated accumulation
 ``eval_aplus_dd_list`` the same loop at listed nodes, exact products and a
-                       double-double sum (Q rounded once: the conditioning-free checker)
+                       double-double sum (Q rounded once, whatever the conditioning)
 ``iphi``               IphiDGV (dgvtools_mod.f90:847-877)
 ``iphi_decomp``        IphiDGV_decomp (dgvtools_mod.f90:896-927), given fM (numpy)
 ``full_aarray``        the canonical records replicated to every node's basis function

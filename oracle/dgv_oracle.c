@@ -1,7 +1,7 @@
 /*
  * dgv_oracle.c -- CPU restatement of the reference's direct DGV collision path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by tests/,
+ * TEST INFRASTRUCTURE ONLY (the oracle, never the product): loaded by tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  See dgv_oracle.h.
  *
  * Arithmetic follows the reference's Fortran expressions term by term (same
@@ -421,7 +421,7 @@ int orc_eval_aplus_comp(const orc_geom* G, const int32_t* capphi, const double* 
 /* a1 at a list of nodes with every term and the sum carried in double-double:
  * each product f(xi') f(xi1') A is formed exactly (two-product with fma), and
  * the terms are added with two-sum into a (hi, lo) pair, so the result is Q
- * rounded once, whatever the conditioning.  The checker for kernels whose
+ * rounded once, whatever the conditioning.  The reference value for kernels whose
  * distance from the reference's sequential FP64 sum is dominated by that sum's
  * own rounding.  abs_sum[t] = sum |terms| / w (the terms' scale). */
 static inline void two_prod(double a, double b, double* p, double* e) {

@@ -1,7 +1,7 @@
 /*
  * dgv_oracle.h -- CPU restatement of the reference's direct DGV collision path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: only tests/, the
  * smoke() entry of __graft_entry__.py and the cpu_baseline leg of bench.py may
  * load it.  The product (dgv0d3v_direct_collision_amd) never links or calls it.
  *

@@ -1,7 +1,7 @@
 """FP64 parity of the single-VDF evaluator on the single-VDF and precision
 configurations' own grids: 27^3 (3 Gauss nodes x 9 cells on [-4,4]^3) with the
 seeded bimodal + 5% perturbation VDF, and 33^3 (11 cells on [-4.5,4.5]^3) with a
-near-equilibrium VDF.  The checker is the oracle at a node sample that covers
+near-equilibrium VDF.  The reference values come from the oracle at a node sample that covers
 every (or every third) canonical basis function in corner, edge, face and
 interior cells:
   * eval_aplus_list: the reference loop and its sequential FP64 sum
