@@ -606,6 +606,15 @@ int dgv_ctx_get_regime(dgv_ctx* ctx, int32_t* run_mode, double* l1_err, double* 
   return DGV_OK;
 }
 
+// The Iphi integrals are the plain sums over the records: the conservation
+// projection that the context may apply to every Q is held off while they run.
+struct ProjectionOff {
+  dgv_ctx* ctx;
+  int save;
+  explicit ProjectionOff(dgv_ctx* c) : ctx(c), save(c->project) { c->project = 0; }
+  ~ProjectionOff() { ctx->project = save; }
+};
+
 // IphiDGV (dgvtools_mod.f90:847-877): for every node i, one basis function per
 // node, Iphi(i) = 2 sum over its records of A f(xi) f(xi1) / w_i, times coef.
 // On the shifted A-array these are the evaluator's Q of f, scaled.
@@ -613,6 +622,7 @@ int dgv_iphi(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef
   LIN_CHECK(ctx, n);
   if (!f || !iphi) return set_error(ctx, DGV_ERR_ARG, "null array");
   const size_t nb = sizeof(double) * n;
+  const ProjectionOff plain(ctx);
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
   int rc = eval_q_dev(ctx, ctx->d_f, ctx->d_q, coef);
@@ -631,6 +641,7 @@ int dgv_iphi_decomp(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, doub
   if (!f || !iphi) return set_error(ctx, DGV_ERR_ARG, "null array");
   const size_t nb = sizeof(double) * n;
   const Grid& g = ctx->g;
+  const ProjectionOff plain(ctx);
   DGV_HIP(ctx, hipSetDevice(ctx->device));
   DGV_HIP(ctx, hipMemcpyAsync(ctx->d_f, f, nb, hipMemcpyHostToDevice, ctx->stream));
   double mom[5];

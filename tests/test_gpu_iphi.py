@@ -7,7 +7,9 @@ node, as the reference's IphiDGV walks them):
    oracle's local Maxwellian (MassCheckRec, maxwelveldist), relative L2 <= 1e-12
    of the result's norm; and, as the split is exact, the difference of two
    IphiDGV evaluations, Iphi(f) - Iphi(fM);
- * the coefficient (mol_diam/L_inf)^2 N_inf scales both.
+ * the coefficient (mol_diam/L_inf)^2 N_inf scales both;
+ * the conservation projection, when enabled for Q, leaves both bit-identical
+   (the reference's IphiDGV is the plain sum) while it does change Q.
 """
 import numpy as np
 import pytest
@@ -44,5 +46,17 @@ def test_iphi_and_decomposition_against_full_a():
               f"{rel_l2(got_d, diff):.2e}", flush=True)
         assert rel_l2(got_d, want_d) <= 1e-12
         assert rel_l2(got_d, diff) <= 1e-10
+        q_plain = d.EvalCollisionPeriodicAPlus_DGV(f)
+        d.set_projection(True)
+        try:
+            q_proj = d.EvalCollisionPeriodicAPlus_DGV(f)
+            assert rel_l2(q_proj, q_plain) > 1e-8
+            np.testing.assert_array_equal(d.IphiDGV(f), got)
+            np.testing.assert_array_equal(d.IphiDGV_decomp(f), got_d)
+            print(f"[iphi] with the projection on: Q moves by {rel_l2(q_proj, q_plain):.2e}, Iphi unchanged",
+                  flush=True)
+        finally:
+            d.set_projection(False)
+        np.testing.assert_array_equal(d.EvalCollisionPeriodicAPlus_DGV(f), q_plain)
     finally:
         d.close()
