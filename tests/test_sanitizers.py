@@ -17,7 +17,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST_TESTS = ["test_aarray_io.py", "test_capi_host.py", "test_driver_io.py", "test_oracle_pins.py",
               "test_oracle_lists.py", "test_oracle_trunc.py", "test_oracle_mixed.py", "test_oracle_regime.py",
-              "test_agen_host.py", "test_close_order.py"]
+              "test_agen_host.py", "test_close_order.py", "test_golden_ref.py", "test_ref_io.py"]
 
 
 def _asan_runtime():
