@@ -411,7 +411,8 @@ class DGV:
 
     def IphiDGV(self, f) -> np.ndarray:
         """dgvtools_mod.f90:847-877: 2/w_i sum A f(xi) f(xi1) for every node's basis
-        function, times (mol_diam/L_inf)^2 N_inf."""
+        function, times (mol_diam/L_inf)^2 N_inf; the plain sums, whatever the
+        projection setting."""
         f = self._f(f)
         out = np.zeros(self.nnodes)
         self._chk(self._L.dgv_iphi(self._ctx, dptr(f), dptr(out), self.nnodes, self.coef))

@@ -241,8 +241,8 @@ int dgv_ctx_get_regime(dgv_ctx* ctx, int32_t* run_mode, double* l1_err, double* 
  * reference's (mol_diam/L_inf)^2 N_inf) -- on the shifted A-array, Q(f) scaled.
  * IphiDGV_decomp(f) (dgvtools_mod.f90:896-927): the same with f split at its
  * local Maxwellian fM (MassCheckRec, maxwelveldist):
- * 2/w_i sum A (fM Df + Df fM + Df Df), Df = f - fM.  The context's projection
- * setting applies to the Q evaluations (off by default, as the reference). */
+ * 2/w_i sum A (fM Df + Df fM + Df Df), Df = f - fM.  Both are the plain record
+ * sums, as in the reference: the context's projection setting does not apply. */
 int dgv_iphi(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef);
 int dgv_iphi_decomp(dgv_ctx* ctx, const double* f, double* iphi, int64_t n, double coef);
 /* CheckSolutionMode_DGV(f,Df) (spat_oper_mod.f90:195-272): moments, local
