@@ -594,6 +594,8 @@ constexpr int kI8BA = 38;                   // coefficient bits: |At| < 2^38
 constexpr int kI8BX = 30;                   // product bits: |x| < 2^30
 constexpr int kI8FS = 15;                   // f scaled into [2^14, 2^15)
 constexpr double kI8Magic = 4503599627370496.0 + 2155905152.0;   // 2^52 + 0x80808080
+static_assert(kI8BX == 2 * kI8FS, "a product of two scaled f values has 2 kI8FS bits");
+static_assert(kI8BX <= 30, "|x| < 2^30 keeps x + 0x80808080 within the low 32 bits of the rounded sum");
 
 template <int D>
 constexpr int i8_dwords_per_pair() { return ((D + 15) / 16) * 16 * 2; }   // 2 dwords per tile row
