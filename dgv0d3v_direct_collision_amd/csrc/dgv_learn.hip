@@ -270,30 +270,51 @@ __device__ __forceinline__ float adam_update(float g, float& w, float& m, float&
   return w;
 }
 
-// Adam on a weight matrix [O][I] (64 x 64 tiles); writes the master weights,
-// the moments and the BF16 copies Wf [O][I] and Wb [I][O].
+// Adam on a weight matrix [O][I] (64 x 64 tiles; O and I are multiples of 64);
+// writes the master weights, the moments and the BF16 copies Wf [O][I] and
+// Wb [I][O].  A thread owns four consecutive columns of four rows: 16-byte
+// loads and stores, and all sixteen loads in flight before the first update.
 __global__ __launch_bounds__(256) void k_adam_w(int O, int I, const float* __restrict__ G, float* __restrict__ W,
                                                 float* __restrict__ m, float* __restrict__ v,
                                                 uint16_t* __restrict__ Wf, uint16_t* __restrict__ Wb, AdamArgs h) {
-  __shared__ uint16_t tile[64][66];
+  // [column][row] of the tile's BF16 weights; rows of 68 halves (136 bytes)
+  // keep the 8-byte reads of the transposed write aligned
+  __shared__ alignas(8) uint16_t tile[64][68];
   const int i0 = blockIdx.x * 64, o0 = blockIdx.y * 64;
-  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
-  for (int k = 0; k < 16; ++k) {
-    const int r = rg + 4 * k;
-    const int64_t idx = (int64_t)(o0 + r) * I + i0 + c;
-    float w = W[idx], mm = m[idx], vv = v[idx];
-    adam_update(G[idx], w, mm, vv, h);
-    W[idx] = w;
-    m[idx] = mm;
-    v[idx] = vv;
-    const uint16_t hb = f2bf(w);
-    Wf[idx] = hb;
-    tile[c][r] = hb;
+  const int c4 = threadIdx.x & 15, rg = threadIdx.x >> 4;   // columns 4 c4 .. 4 c4 + 3, rows rg + 16 k
+  float4 g[4], w[4], mm[4], vv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t idx = (int64_t)(o0 + rg + 16 * k) * I + i0 + 4 * c4;
+    g[k] = *reinterpret_cast<const float4*>(G + idx);
+    w[k] = *reinterpret_cast<const float4*>(W + idx);
+    mm[k] = *reinterpret_cast<const float4*>(m + idx);
+    vv[k] = *reinterpret_cast<const float4*>(v + idx);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = rg + 16 * k;
+    const int64_t idx = (int64_t)(o0 + r) * I + i0 + 4 * c4;
+    adam_update(g[k].x, w[k].x, mm[k].x, vv[k].x, h);
+    adam_update(g[k].y, w[k].y, mm[k].y, vv[k].y, h);
+    adam_update(g[k].z, w[k].z, mm[k].z, vv[k].z, h);
+    adam_update(g[k].w, w[k].w, mm[k].w, vv[k].w, h);
+    *reinterpret_cast<float4*>(W + idx) = w[k];
+    *reinterpret_cast<float4*>(m + idx) = mm[k];
+    *reinterpret_cast<float4*>(v + idx) = vv[k];
+    const uint16_t b0 = f2bf(w[k].x), b1 = f2bf(w[k].y), b2 = f2bf(w[k].z), b3 = f2bf(w[k].w);
+    *reinterpret_cast<uint2*>(Wf + idx) = make_uint2((uint32_t)b0 | ((uint32_t)b1 << 16), (uint32_t)b2 | ((uint32_t)b3 << 16));
+    tile[4 * c4 + 0][r] = b0;
+    tile[4 * c4 + 1][r] = b1;
+    tile[4 * c4 + 2][r] = b2;
+    tile[4 * c4 + 3][r] = b3;
   }
   __syncthreads();
-  for (int k = 0; k < 16; ++k) {
-    const int rr = rg + 4 * k;
-    Wb[(int64_t)(i0 + rr) * O + o0 + c] = tile[rr][c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ii = rg + 16 * k;   // column i0 + ii: rows o0 + 4 c4 .. o0 + 4 c4 + 3
+    *reinterpret_cast<uint2*>(Wb + (int64_t)(i0 + ii) * O + o0 + 4 * c4) =
+        *reinterpret_cast<const uint2*>(&tile[ii][4 * c4]);
   }
 }
 

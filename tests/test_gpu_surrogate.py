@@ -4,7 +4,8 @@
  * forward output and every weight / bias gradient of one training step, with
    BF16 rounding at the GPU's points and float64 sums on the CPU: relative L2
    <= 2e-3 per tensor (the GPU accumulates in FP32);
- * one Adam step against the textbook update from the same gradients;
+ * one Adam step against the textbook update from the same gradients, and the
+   BF16 weight copies it writes against those made from its master weights;
  * the FP64 projection: sum_i w_i psi_k q_i <= 1e-13 of sum_i |w_i psi_k q_i|;
  * training on (f, Q(f)) pairs from the direct evaluator: the held-out error
    falls far below that of the method without a surrogate (Q taken as 0);
@@ -111,8 +112,22 @@ def test_adam_step_matches_cpu(grid):
         eW, _, _ = sr.adam(Ws[l].astype(np.float64), 0.0, 0.0, grads[l][0].astype(np.float64), 1, 1e-3)
         eb, _, _ = sr.adam(bs[l].astype(np.float64), 0.0, 0.0, grads[l][1].astype(np.float64), 1, 1e-3)
         assert np.max(np.abs(W1 - eW)) <= 2e-6 and np.max(np.abs(b1 - eb)) <= 2e-6
+    # the BF16 copies the Adam step wrote (forward [O][I] and transposed [I][O])
+    # equal those made from the updated master weights by the conversion kernel:
+    # forward outputs and every gradient bit for bit
+    s3 = dgv.Surrogate(grid, widths, batch=64, seed=1)
+    for l in range(len(Ws)):
+        s3.set_layer(l, *s2.get_layer(l))
+    s3.set_scales(0.2, 0.05)
+    np.testing.assert_array_equal(s2.predict_host(f, project=False), s3.predict_host(f, project=False))
+    s2.train_step_host(f, q, lr=0.0)
+    s3.train_step_host(f, q, lr=0.0)
+    for l in range(len(Ws)):
+        for a, b in zip(s2.get_grad(l), s3.get_grad(l)):
+            np.testing.assert_array_equal(a, b)
     s.close()
     s2.close()
+    s3.close()
 
 
 def test_projection_conserves_to_1e13(grid):
